@@ -1,0 +1,215 @@
+/*
+ * lgh.h -- C ABI of the MI355X (gfx950) statevector library behind the
+ * lightning.gpu device.
+ *
+ * The reference crosses from its pybind11 module into closed NVIDIA
+ * libraries (cuStateVec, cuBLAS, cuSPARSE) inside StateVectorCudaManaged;
+ * this header is the equivalent seam of the MI355X build: the C++ host class
+ * StateVectorHIPManaged and the pybind11 module `lightning_gpu_qubit_ops`
+ * call only these functions, and every function is implemented with
+ * hand-written HIP kernels.  Each declaration cites the reference interface
+ * it replaces (paths relative to the reference repository root; "SVCM" is
+ * pennylane_lightning_gpu/src/simulator/StateVectorCudaManaged.hpp).
+ *
+ * Conventions
+ *  - Wire w of an n-qubit state is amplitude-index bit (n-1-w) (SVCM:1349-1356).
+ *  - Matrices are row-major, interleaved (re, im) doubles; the first listed
+ *    wire is the most significant bit of the matrix index (PennyLane order).
+ *  - Complex host buffers of states are interleaved (re, im) in the state's
+ *    precision: float for LGH_C64, double for LGH_C128.
+ *  - Every function returns LGH_OK or an error code; lgh_last_error() returns
+ *    the message of the calling thread's last failure.
+ *  - Work on one state is ordered on that state's HIP stream.  Functions that
+ *    return host values synchronise that stream.  A handle must not be used
+ *    from two threads at once.
+ */
+#ifndef LGH_H
+#define LGH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGH_ABI_VERSION 1
+
+typedef enum lgh_status {
+    LGH_OK = 0,
+    LGH_ERR_INVALID_ARGUMENT = 1,
+    LGH_ERR_UNSUPPORTED = 2,
+    LGH_ERR_OUT_OF_MEMORY = 3,
+    LGH_ERR_DEVICE = 4,
+    LGH_ERR_NO_DEVICE = 5
+} lgh_status;
+
+typedef enum lgh_dtype { LGH_C64 = 0, LGH_C128 = 1 } lgh_dtype;
+
+/* Gate identifiers: the named methods of LightningGPU_C{64,128}
+ * (pennylane_lightning_gpu/src/bindings/Bindings.cpp:120-427) plus an
+ * explicit-matrix gate (Bindings.cpp:258-262, SVCM:226-245). */
+typedef enum lgh_gate {
+    LGH_GATE_STATEPREP = -1, /* state preparation record: skipped by the adjoint sweep */
+    LGH_GATE_IDENTITY = 0,
+    LGH_GATE_PAULIX,
+    LGH_GATE_PAULIY,
+    LGH_GATE_PAULIZ,
+    LGH_GATE_HADAMARD,
+    LGH_GATE_S,
+    LGH_GATE_T,
+    LGH_GATE_CNOT,
+    LGH_GATE_SWAP,
+    LGH_GATE_CSWAP,
+    LGH_GATE_TOFFOLI,
+    LGH_GATE_CY,
+    LGH_GATE_CZ,
+    LGH_GATE_PHASESHIFT,
+    LGH_GATE_CONTROLLEDPHASESHIFT,
+    LGH_GATE_RX,
+    LGH_GATE_RY,
+    LGH_GATE_RZ,
+    LGH_GATE_ROT,
+    LGH_GATE_CRX,
+    LGH_GATE_CRY,
+    LGH_GATE_CRZ,
+    LGH_GATE_CROT,
+    LGH_GATE_ISINGXX,
+    LGH_GATE_ISINGYY,
+    LGH_GATE_ISINGZZ,
+    LGH_GATE_SINGLEEXCITATION,
+    LGH_GATE_SINGLEEXCITATIONMINUS,
+    LGH_GATE_SINGLEEXCITATIONPLUS,
+    LGH_GATE_DOUBLEEXCITATION,
+    LGH_GATE_DOUBLEEXCITATIONMINUS,
+    LGH_GATE_DOUBLEEXCITATIONPLUS,
+    LGH_GATE_MULTIRZ,
+    LGH_GATE_MATRIX, /* op.matrix on the last (num_wires - num_ctrl) wires */
+    LGH_GATE_COUNT
+} lgh_gate;
+
+/* One operation.  `wires` lists controls first, as the reference's
+ * ctrl_map does (pennylane_lightning_gpu/src/simulator/StateVectorCudaBase.hpp:288-312).
+ * Named gates read params[0] (Rot/CRot read all three). */
+typedef struct lgh_op {
+    int32_t gate;          /* lgh_gate */
+    int32_t num_wires;
+    int32_t num_ctrl;      /* LGH_GATE_MATRIX only: leading control wires */
+    int32_t inverse;       /* apply the adjoint */
+    const int32_t *wires;  /* host pointer, num_wires entries */
+    double params[3];
+    const double *matrix;  /* LGH_GATE_MATRIX: 2^k x 2^k complex, k = num_wires - num_ctrl */
+    int32_t num_params;    /* parameters the op carries (adjoint bookkeeping) */
+    int32_t reserved;
+} lgh_op;
+
+typedef struct lgh_sv lgh_sv;   /* device-resident state vector */
+typedef struct lgh_csr lgh_csr; /* device-resident CSR matrix */
+
+/* ---- errors / version ------------------------------------------------ */
+const char *lgh_last_error(void);
+int lgh_abi_version(void);
+
+/* ---- devices (Bindings.cpp:1736-1777; cuda_helpers.hpp:627-631) ------ */
+/* GetNumGPUs / DevPool.getTotalDevices */
+lgh_status lgh_device_count(int *count);
+/* get_gpu_arch: writes the gfx target name, e.g. "gfx950" */
+lgh_status lgh_device_arch(int device, char *name, size_t len);
+/* is_gpu_supported: 1 for gfx950 */
+lgh_status lgh_device_supported(int device, int *supported);
+lgh_status lgh_device_memory(int device, uint64_t *free_bytes, uint64_t *total_bytes);
+lgh_status lgh_device_synchronize(int device);
+/* device_reset: frees cached buffers, then resets the device */
+lgh_status lgh_device_reset(void);
+/* Returns cached device buffers of `device` to the runtime. */
+lgh_status lgh_pool_trim(int device);
+/* Streams for callers that overlap independent states (serving). */
+lgh_status lgh_stream_create(int device, void **stream);
+lgh_status lgh_stream_destroy(void *stream);
+lgh_status lgh_stream_synchronize(void *stream);
+
+/* ---- state vectors (SVCM:84-185, StateVectorCudaBase.hpp:104-240) ---- */
+/* StateVectorCudaManaged(num_qubits[, DevTag]); stream NULL = library stream
+ * of `device`.  The state is left uninitialised. */
+lgh_status lgh_sv_create(lgh_sv **out, int num_qubits, lgh_dtype dtype, int device, void *stream);
+lgh_status lgh_sv_destroy(lgh_sv *sv);
+lgh_status lgh_sv_info(const lgh_sv *sv, int *num_qubits, lgh_dtype *dtype, int *device,
+                       void **stream, void **device_data);
+/* initSV / resetGPU: |0...0> (StateVectorCudaBase.hpp:234-240) */
+lgh_status lgh_sv_reset(lgh_sv *sv);
+/* setBasisState: zero, then amplitude `index` = value (SVCM:144-152) */
+lgh_status lgh_sv_set_basis(lgh_sv *sv, uint64_t index, double re, double im);
+/* setStateVector: zero, then sv[indices[i]] = values[i] (SVCM:164-185,
+ * initSV.cu:64-97).  values: complex in the state's precision. */
+lgh_status lgh_sv_set_indexed(lgh_sv *sv, const int64_t *indices, const void *values, uint64_t count);
+/* CopyHostDataToGpu (StateVectorCudaBase.hpp:104-130) */
+lgh_status lgh_sv_copy_h2d(lgh_sv *sv, const void *host, uint64_t count);
+/* CopyGpuDataToHost (StateVectorCudaBase.hpp:131-170) */
+lgh_status lgh_sv_copy_d2h(const lgh_sv *sv, void *host, uint64_t count);
+/* DeviceToDevice / updateData (StateVectorCudaBase.hpp:205-219) */
+lgh_status lgh_sv_copy_d2d(lgh_sv *dst, const lgh_sv *src);
+lgh_status lgh_sv_synchronize(const lgh_sv *sv);
+
+/* ---- gates (SVCM:198-688) ------------------------------------------- */
+/* applyOperation for one op or a sequence (SVCM:198-247, :279-315). */
+lgh_status lgh_apply(lgh_sv *sv, const lgh_op *ops, int64_t num_ops);
+/* Host-only utility (no device needed): the 2^w x 2^w matrix, controls
+ * included and wires in op order, that lgh_apply applies for `op`; complex
+ * (re, im) row-major into `out`. */
+lgh_status lgh_gate_matrix(const lgh_op *op, double *out);
+/* Generator of a differentiable gate, unscaled (GateGenerators.hpp:58-320,
+ * SVCM:562-688).  Writes the generator's scaling factor to *scale. */
+lgh_status lgh_apply_generator(lgh_sv *sv, const lgh_op *op, double *scale);
+
+/* ---- BLAS-like helpers (cuda_helpers.hpp:456-560) -------------------- */
+/* innerProdC_CUDA: <a|b> */
+lgh_status lgh_inner(const lgh_sv *a, const lgh_sv *b, double *re, double *im);
+/* scaleAndAddC_CUDA: y += alpha * x */
+lgh_status lgh_axpy(lgh_sv *y, double alpha_re, double alpha_im, const lgh_sv *x);
+/* scaleC_CUDA: x *= alpha */
+lgh_status lgh_scale(lgh_sv *x, double alpha_re, double alpha_im);
+
+/* ---- measurements (SVCM:702-1148) ------------------------------------ */
+/* expval(wires, matrix) / getExpectationValueDeviceMatrix (SVCM:755-780, :1649-1713) */
+lgh_status lgh_expval_matrix(const lgh_sv *sv, const int32_t *wires, int32_t num_wires,
+                             const double *matrix, double *re, double *im);
+/* getExpectationValuePauliWords (SVCM:1071-1148).  Word w is
+ * paulis[offsets[w]..offsets[w+1]) over {'I','X','Y','Z'} on the wires at the
+ * same positions; coeffs are complex (re, im).  *result = Re(sum_w c_w <P_w>);
+ * per_word (nullable) receives <P_w>. */
+lgh_status lgh_expval_pauli_words(const lgh_sv *sv, int64_t num_words, const char *paulis,
+                                  const int32_t *wires, const int64_t *offsets,
+                                  const double *coeffs, double *per_word, double *result);
+/* CSR kept resident on the device (replaces the per-call upload of
+ * getExpectationValueOnSparseSpMV, SVCM:795-922).  values: complex128. */
+lgh_status lgh_csr_create(lgh_csr **out, int device, lgh_dtype dtype, int64_t num_rows,
+                          const int64_t *indptr, const int64_t *indices, const double *values,
+                          int64_t nnz);
+lgh_status lgh_csr_destroy(lgh_csr *csr);
+/* <psi|H|psi> with SpMV and dot fused (SVCM:854-919) */
+lgh_status lgh_expval_csr(const lgh_sv *sv, const lgh_csr *csr, double *re, double *im);
+/* out = H in (SparseHamiltonianGPU::applyInPlace, ObservablesGPU.hpp:469-587) */
+lgh_status lgh_apply_csr(lgh_sv *out, const lgh_csr *csr, const lgh_sv *in);
+/* probability(wires): wires[0] is the least significant output bit, as the
+ * reference returns it (SVCM:931-970).  out: 2^num_wires doubles. */
+lgh_status lgh_probs(const lgh_sv *sv, const int32_t *wires, int32_t num_wires, double *out);
+/* generate_samples (SVCM:982-1061) with a seeded Philox4x32-10 stream:
+ * writes `shots` basis indices in ascending order. */
+lgh_status lgh_sample(const lgh_sv *sv, uint64_t shots, uint64_t seed, uint64_t *indices);
+/* Same, expanded to a (shots x num_qubits) 0/1 array: column (n-1-j) holds
+ * bit j of the index (SVCM:1037-1055). */
+lgh_status lgh_sample_bits(const lgh_sv *sv, uint64_t shots, uint64_t seed, uint64_t *bits);
+
+/* ---- adjoint Jacobian (AdjointDiffGPU.hpp:499-596) -------------------- */
+/* psi: forward state (read only; copied).  bras[j] = O_j psi on entry,
+ * overwritten.  trainable: sorted indices into the parametrised ops.
+ * jac: num_obs x num_trainable, row-major.  If apply_operations, psi is the
+ * initial state and the ops are applied first. */
+lgh_status lgh_adjoint_jacobian(const lgh_sv *psi, lgh_sv *const *bras, int32_t num_obs,
+                                const lgh_op *ops, int64_t num_ops, const int64_t *trainable,
+                                int64_t num_trainable, int32_t apply_operations, double *jac);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LGH_H */

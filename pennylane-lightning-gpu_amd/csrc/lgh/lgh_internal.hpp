@@ -1,0 +1,221 @@
+// Internal declarations shared by the HIP translation units of liblgh.so.
+// Nothing here is part of the C ABI (include/lgh.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "lgh.h"
+
+namespace lgh {
+
+// ---------------------------------------------------------------- errors
+
+struct Error : std::runtime_error {
+    lgh_status code;
+    Error(lgh_status c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string &msg);
+
+#define LGH_HIP(expr)                                                                   \
+    do {                                                                                \
+        hipError_t lgh_e_ = (expr);                                                     \
+        if (lgh_e_ != hipSuccess)                                                       \
+            throw ::lgh::Error(lgh_e_ == hipErrorOutOfMemory ? LGH_ERR_OUT_OF_MEMORY    \
+                                                             : LGH_ERR_DEVICE,          \
+                               std::string(#expr) + ": " + hipGetErrorString(lgh_e_)); \
+    } while (0)
+
+#define LGH_REQUIRE(cond, msg)                                                  \
+    do {                                                                        \
+        if (!(cond)) throw ::lgh::Error(LGH_ERR_INVALID_ARGUMENT, (msg));       \
+    } while (0)
+
+#define LGH_LAUNCH_CHECK() LGH_HIP(hipGetLastError())
+
+template <class F> lgh_status guard(F &&f) {
+    try {
+        f();
+        return LGH_OK;
+    } catch (const Error &e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_last_error("host allocation failed");
+        return LGH_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        set_last_error(e.what());
+        return LGH_ERR_INVALID_ARGUMENT;
+    }
+}
+
+// ---------------------------------------------------------------- device types
+
+// Interleaved complex amplitude; 16-byte aligned for complex128 so that one
+// amplitude is one global_load_dwordx4.
+template <class R> struct alignas(2 * sizeof(R)) cx {
+    R re, im;
+};
+
+template <class R> __host__ __device__ __forceinline__ cx<R> mk(R a, R b) { return cx<R>{a, b}; }
+template <class R> __device__ __forceinline__ cx<R> cadd(cx<R> a, cx<R> b) {
+    return cx<R>{a.re + b.re, a.im + b.im};
+}
+template <class R> __device__ __forceinline__ cx<R> cmul(cx<R> a, cx<R> b) {
+    return cx<R>{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+// a * b + c
+template <class R> __device__ __forceinline__ cx<R> cfma(cx<R> a, cx<R> b, cx<R> c) {
+    return cx<R>{c.re + a.re * b.re - a.im * b.im, c.im + a.re * b.im + a.im * b.re};
+}
+
+template <class R> cx<R> to_cx(std::complex<double> z) {
+    return cx<R>{static_cast<R>(z.real()), static_cast<R>(z.imag())};
+}
+
+constexpr int kMaxFixed = 16; // target + control bits that a kernel enumerates around
+constexpr int kMaxRegK = 4;   // dense gates held in registers up to 4 targets
+constexpr int kMaxLdsK = 13;  // dense gates staged through LDS up to 13 targets
+
+// Bit positions (ascending) where a zero is inserted into a group index.
+struct Fixed {
+    int cnt;
+    int pos[kMaxFixed];
+};
+
+__device__ __forceinline__ uint64_t expand(uint64_t g, const Fixed &f) {
+#pragma unroll 4
+    for (int k = 0; k < f.cnt; ++k) {
+        const int p = f.pos[k];
+        const uint64_t lo = g & ((1ull << p) - 1ull);
+        g = ((g >> p) << (p + 1)) | lo;
+    }
+    return g;
+}
+
+Fixed make_fixed(std::vector<int> bits);
+
+// ---------------------------------------------------------------- runtime objects
+
+// Per (device, stream) context: the stream, a grow-only device workspace,
+// a pinned host buffer for results and a pinned upload ring for small
+// operands (gate matrices, pointer tables).
+struct StreamCtx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool owns_stream = false;
+    int refs = 0;
+
+    void *ws = nullptr;   // device workspace
+    size_t ws_bytes = 0;
+    void *host = nullptr; // pinned host staging for results
+    size_t host_bytes = 0;
+
+    void *up_host = nullptr; // pinned upload ring (host side)
+    void *up_dev = nullptr;  // device side of the ring
+    size_t up_bytes = 0;
+    size_t up_off = 0;
+
+    void *workspace(size_t bytes);
+    void *host_buffer(size_t bytes);
+    // Copies `bytes` from host memory into the device ring (stream ordered)
+    // and returns the device address.  Valid until the ring wraps, which
+    // synchronises the stream first.
+    void *upload(const void *src, size_t bytes);
+    void sync();
+};
+
+StreamCtx *acquire_ctx(int device, hipStream_t stream);
+void release_ctx(StreamCtx *ctx);
+
+// Caching device allocator (exact-size free lists per device).
+void *pool_alloc(int device, size_t bytes);
+void pool_free(int device, void *ptr, size_t bytes);
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev);
+    ~DeviceGuard();
+};
+
+int sm_count(int device);
+
+} // namespace lgh
+
+struct lgh_sv {
+    int n = 0;
+    lgh_dtype dtype = LGH_C128;
+    int device = 0;
+    lgh::StreamCtx *ctx = nullptr;
+    void *data = nullptr;
+    size_t bytes = 0;
+    uint64_t length() const { return 1ull << n; }
+    size_t elem() const { return dtype == LGH_C128 ? 16 : 8; }
+};
+
+struct lgh_csr {
+    int device = 0;
+    lgh_dtype dtype = LGH_C128;
+    int64_t rows = 0, nnz = 0;
+    bool idx64 = true;
+    void *indptr = nullptr;  // int64 or int32, rows + 1
+    void *indices = nullptr; // int64 or int32, nnz
+    void *values = nullptr;  // cx<R>, nnz
+    size_t b_ptr = 0, b_idx = 0, b_val = 0;
+    double avg_row = 0;
+};
+
+namespace lgh {
+
+// ---------------------------------------------------------------- compiled operations
+
+// A gate reduced to one of two kernel shapes.
+//  DENSE: a 2^k x 2^k matrix on target bits tbits[0..k) (tbits[0] is the most
+//         significant bit of the matrix index) applied where all cmask bits are 1.
+//  DIAG:  amplitudes with all cmask bits set are multiplied by d_even or d_odd
+//         according to the parity of (index & pmask).
+struct KOp {
+    enum Kind { NOP, DENSE, DIAG } kind = NOP;
+    int k = 0;
+    int tbits[kMaxLdsK] = {};
+    uint64_t cmask = 0;
+    std::vector<int> cbits;
+    std::vector<std::complex<double>> mat;
+    uint64_t pmask = 0;
+    std::complex<double> d_even{1, 0}, d_odd{1, 0};
+};
+
+// Generator data aligned with a KOp of the same op: for DENSE a 2^k x 2^k
+// matrix on the same targets (restricted to the control subspace), for DIAG
+// real eigenvalues by parity.
+struct KGen {
+    bool present = false;
+    std::vector<std::complex<double>> mat;
+    double g_even = 0, g_odd = 0;
+    double scale = 0;
+};
+
+void validate_op(const lgh_op &op, int n);
+// U (or U^dagger when `dagger`) of op as a kernel op.
+KOp compile_op(const lgh_op &op, int n, bool dagger);
+// Generator of op matched to compile_op's kernel shape.
+KGen compile_generator(const lgh_op &op, int n, const KOp &shape);
+// Full matrix over all op wires (controls included) -- host utility.
+std::vector<std::complex<double>> full_matrix(const lgh_op &op, bool dagger);
+
+// Launch a compiled op on one state.
+void launch_kop(lgh_sv *sv, const KOp &op);
+
+// Device-side reduction helpers (implemented in lgh_measure.hip).
+// Sums `count` partial values (stride 1) into out_dev[0] with a fixed order.
+void launch_reduce(StreamCtx *ctx, const double *partials, int nparts, int nvals, double *out_dev);
+
+} // namespace lgh

@@ -1,0 +1,331 @@
+"""ctypes view of the C ABI (``include/lgh.h``) exported by ``liblgh.so``.
+
+The pybind11 module ``lightning_gpu_qubit_ops`` is the drop-in boundary the
+device class uses; this module gives Python direct access to the same C ABI
+for callers that want it (the batching and serving forms, the torch autograd
+function, and the C-ABI tests).  There is no fallback: if ``liblgh.so`` is
+missing, loading fails with an error.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblgh.so")
+
+LGH_C64 = 0
+LGH_C128 = 1
+
+GATES = [
+    "Identity", "PauliX", "PauliY", "PauliZ", "Hadamard", "S", "T", "CNOT", "SWAP", "CSWAP",
+    "Toffoli", "CY", "CZ", "PhaseShift", "ControlledPhaseShift", "RX", "RY", "RZ", "Rot", "CRX",
+    "CRY", "CRZ", "CRot", "IsingXX", "IsingYY", "IsingZZ", "SingleExcitation",
+    "SingleExcitationMinus", "SingleExcitationPlus", "DoubleExcitation", "DoubleExcitationMinus",
+    "DoubleExcitationPlus", "MultiRZ", "Matrix",
+]
+GATE_ID = {name: i for i, name in enumerate(GATES)}
+GATE_STATEPREP = -1
+PARAM_COUNT = {"Rot": 3, "CRot": 3}
+for _g in ("PhaseShift", "ControlledPhaseShift", "RX", "RY", "RZ", "CRX", "CRY", "CRZ", "IsingXX",
+           "IsingYY", "IsingZZ", "SingleExcitation", "SingleExcitationMinus", "SingleExcitationPlus",
+           "DoubleExcitation", "DoubleExcitationMinus", "DoubleExcitationPlus", "MultiRZ"):
+    PARAM_COUNT[_g] = 1
+
+
+class LghError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lgh error {code}: {msg}")
+        self.code = code
+
+
+class Op(C.Structure):
+    _fields_ = [
+        ("gate", C.c_int32),
+        ("num_wires", C.c_int32),
+        ("num_ctrl", C.c_int32),
+        ("inverse", C.c_int32),
+        ("wires", C.POINTER(C.c_int32)),
+        ("params", C.c_double * 3),
+        ("matrix", C.POINTER(C.c_double)),
+        ("num_params", C.c_int32),
+        ("reserved", C.c_int32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with build_ext.py")
+        L = C.CDLL(LIB_PATH)
+        L.lgh_last_error.restype = C.c_char_p
+        vp = C.c_void_p
+        L.lgh_sv_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int, vp]
+        L.lgh_sv_destroy.argtypes = [vp]
+        L.lgh_sv_info.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(vp), C.POINTER(vp)]
+        L.lgh_sv_reset.argtypes = [vp]
+        L.lgh_sv_set_basis.argtypes = [vp, C.c_uint64, C.c_double, C.c_double]
+        L.lgh_sv_set_indexed.argtypes = [vp, vp, vp, C.c_uint64]
+        L.lgh_sv_copy_h2d.argtypes = [vp, vp, C.c_uint64]
+        L.lgh_sv_copy_d2h.argtypes = [vp, vp, C.c_uint64]
+        L.lgh_sv_copy_d2d.argtypes = [vp, vp]
+        L.lgh_sv_synchronize.argtypes = [vp]
+        L.lgh_apply.argtypes = [vp, C.POINTER(Op), C.c_int64]
+        L.lgh_gate_matrix.argtypes = [C.POINTER(Op), vp]
+        L.lgh_apply_generator.argtypes = [vp, C.POINTER(Op), C.POINTER(C.c_double)]
+        L.lgh_inner.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.lgh_axpy.argtypes = [vp, C.c_double, C.c_double, vp]
+        L.lgh_scale.argtypes = [vp, C.c_double, C.c_double]
+        L.lgh_expval_matrix.argtypes = [vp, vp, C.c_int32, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.lgh_expval_pauli_words.argtypes = [vp, C.c_int64, C.c_char_p, vp, vp, vp, vp, C.POINTER(C.c_double)]
+        L.lgh_csr_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int64, vp, vp, vp, C.c_int64]
+        L.lgh_csr_destroy.argtypes = [vp]
+        L.lgh_expval_csr.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.lgh_apply_csr.argtypes = [vp, vp, vp]
+        L.lgh_probs.argtypes = [vp, vp, C.c_int32, vp]
+        L.lgh_sample.argtypes = [vp, C.c_uint64, C.c_uint64, vp]
+        L.lgh_sample_bits.argtypes = [vp, C.c_uint64, C.c_uint64, vp]
+        L.lgh_adjoint_jacobian.argtypes = [vp, C.POINTER(vp), C.c_int32, C.POINTER(Op), C.c_int64, vp,
+                                           C.c_int64, C.c_int32, vp]
+        L.lgh_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.lgh_device_arch.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
+        L.lgh_device_supported.argtypes = [C.c_int, C.POINTER(C.c_int)]
+        L.lgh_device_memory.argtypes = [C.c_int, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+        L.lgh_device_synchronize.argtypes = [C.c_int]
+        L.lgh_pool_trim.argtypes = [C.c_int]
+        L.lgh_stream_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.lgh_stream_destroy.argtypes = [vp]
+        L.lgh_stream_synchronize.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise LghError(rc, lib().lgh_last_error().decode())
+
+
+def device_count() -> int:
+    c = C.c_int()
+    check(lib().lgh_device_count(C.byref(c)))
+    return c.value
+
+
+def make_ops(ops):
+    """Pack ``(name, wires, params, inverse[, matrix])`` tuples into an lgh_op array.
+
+    Returns the array and a list of buffers that must outlive the call.
+    """
+    arr = (Op * max(1, len(ops)))()
+    keep = []
+    for i, op in enumerate(ops):
+        name, wires, params, inverse = op[:4]
+        mat = op[4] if len(op) > 4 else None
+        o = arr[i]
+        if name in ("StatePrep", "QubitStateVector", "BasisState"):
+            o.gate = GATE_STATEPREP
+        elif name in GATE_ID and name != "Matrix":
+            o.gate = GATE_ID[name]
+        elif mat is not None:
+            o.gate = GATE_ID["Matrix"]
+        else:
+            raise ValueError(f"Currently unsupported gate: {name}")
+        w = (C.c_int32 * max(1, len(wires)))(*[int(x) for x in wires])
+        keep.append(w)
+        o.wires = C.cast(w, C.POINTER(C.c_int32))
+        o.num_wires = len(wires)
+        o.inverse = int(bool(inverse))
+        for k, p in enumerate(list(params)[:3]):
+            o.params[k] = float(p)
+        o.num_params = len(params)
+        if mat is not None and o.gate == GATE_ID["Matrix"]:
+            m = np.ascontiguousarray(np.asarray(mat, dtype=np.complex128).ravel()).view(np.float64)
+            keep.append(m)
+            o.matrix = m.ctypes.data_as(C.POINTER(C.c_double))
+            o.num_ctrl = 0
+    return arr, keep
+
+
+def gate_matrix(name, wires, params=(), inverse=False, matrix=None):
+    """Host-side matrix the library applies for one op (no device needed)."""
+    op = (name, wires, params, inverse) + ((matrix,) if matrix is not None else ())
+    arr, keep = make_ops([op])
+    dim = 1 << len(wires)
+    out = np.zeros(dim * dim, dtype=np.complex128)
+    check(lib().lgh_gate_matrix(arr, out.ctypes.data))
+    return out.reshape(dim, dim)
+
+
+class State:
+    """Owning handle of one device state (``lgh_sv``)."""
+
+    def __init__(self, num_qubits: int, dtype=np.complex128, device: int = 0, stream=None):
+        self.n = int(num_qubits)
+        self.dtype = np.dtype(dtype)
+        code = LGH_C128 if self.dtype == np.complex128 else LGH_C64
+        h = C.c_void_p()
+        check(lib().lgh_sv_create(C.byref(h), self.n, code, device, stream))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.lgh_sv_destroy(h)
+            self.handle = None
+
+    def close(self):
+        self.__del__()
+
+    @property
+    def length(self):
+        return 1 << self.n
+
+    def data_ptr(self) -> int:
+        p = C.c_void_p()
+        check(lib().lgh_sv_info(self.handle, None, None, None, None, C.byref(p)))
+        return p.value
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        check(lib().lgh_sv_info(self.handle, None, None, None, C.byref(s), None))
+        return s.value or 0
+
+    def reset(self):
+        check(lib().lgh_sv_reset(self.handle))
+        return self
+
+    def set_basis(self, index: int, value=1.0):
+        v = complex(value)
+        check(lib().lgh_sv_set_basis(self.handle, index, v.real, v.imag))
+        return self
+
+    def set_indexed(self, indices, values):
+        idx = np.ascontiguousarray(indices, dtype=np.int64)
+        val = np.ascontiguousarray(values, dtype=self.dtype)
+        check(lib().lgh_sv_set_indexed(self.handle, idx.ctypes.data, val.ctypes.data, len(idx)))
+        return self
+
+    def upload(self, host):
+        h = np.ascontiguousarray(host, dtype=self.dtype)
+        check(lib().lgh_sv_copy_h2d(self.handle, h.ctypes.data, h.size))
+        return self
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.length, dtype=self.dtype)
+        check(lib().lgh_sv_copy_d2h(self.handle, out.ctypes.data, out.size))
+        return out
+
+    def copy_from(self, other: "State"):
+        check(lib().lgh_sv_copy_d2d(self.handle, other.handle))
+        return self
+
+    def synchronize(self):
+        check(lib().lgh_sv_synchronize(self.handle))
+
+    def apply(self, ops):
+        if not ops:
+            return self
+        arr, keep = make_ops(ops)
+        check(lib().lgh_apply(self.handle, arr, len(ops)))
+        return self
+
+    def apply_generator(self, name, wires):
+        arr, keep = make_ops([(name, wires, [0.0] * PARAM_COUNT.get(name, 0), False)])
+        s = C.c_double()
+        check(lib().lgh_apply_generator(self.handle, arr, C.byref(s)))
+        return s.value
+
+    def inner(self, other: "State") -> complex:
+        re, im = C.c_double(), C.c_double()
+        check(lib().lgh_inner(self.handle, other.handle, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def axpy(self, alpha, x: "State"):
+        a = complex(alpha)
+        check(lib().lgh_axpy(self.handle, a.real, a.imag, x.handle))
+
+    def scale(self, alpha):
+        a = complex(alpha)
+        check(lib().lgh_scale(self.handle, a.real, a.imag))
+
+    def expval_matrix(self, wires, M) -> complex:
+        w = np.ascontiguousarray(wires, dtype=np.int32)
+        m = np.ascontiguousarray(np.asarray(M, dtype=np.complex128).ravel())
+        re, im = C.c_double(), C.c_double()
+        check(lib().lgh_expval_matrix(self.handle, w.ctypes.data, len(w), m.ctypes.data, C.byref(re),
+                                      C.byref(im)))
+        return complex(re.value, im.value)
+
+    def expval_pauli_words(self, words, wires, coeffs):
+        chars = "".join(words).encode()
+        wf = np.ascontiguousarray([x for ws in wires for x in ws] or [0], dtype=np.int32)
+        offs = np.zeros(len(words) + 1, dtype=np.int64)
+        offs[1:] = np.cumsum([len(w) for w in words])
+        c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.complex128))
+        per = np.zeros(max(1, len(words)))
+        tot = C.c_double()
+        check(lib().lgh_expval_pauli_words(self.handle, len(words), chars, wf.ctypes.data, offs.ctypes.data,
+                                           c.ctypes.data, per.ctypes.data, C.byref(tot)))
+        return tot.value, per[: len(words)]
+
+    def probs(self, wires) -> np.ndarray:
+        w = np.ascontiguousarray(wires, dtype=np.int32)
+        out = np.zeros(1 << len(w))
+        check(lib().lgh_probs(self.handle, w.ctypes.data, len(w), out.ctypes.data))
+        return out
+
+    def sample(self, shots, seed) -> np.ndarray:
+        out = np.zeros(max(1, shots), dtype=np.uint64)
+        check(lib().lgh_sample(self.handle, shots, seed, out.ctypes.data))
+        return out[:shots]
+
+    def sample_bits(self, shots, seed) -> np.ndarray:
+        out = np.zeros((max(1, shots), self.n), dtype=np.uint64)
+        check(lib().lgh_sample_bits(self.handle, shots, seed, out.ctypes.data))
+        return out[:shots]
+
+
+class CSR:
+    """Device-resident CSR matrix (``lgh_csr``)."""
+
+    def __init__(self, indptr, indices, data, dtype=np.complex128, device=0):
+        self.indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+        self.indices = np.ascontiguousarray(indices, dtype=np.int64)
+        vals = np.ascontiguousarray(np.asarray(data, dtype=np.complex128))
+        h = C.c_void_p()
+        code = LGH_C128 if np.dtype(dtype) == np.complex128 else LGH_C64
+        check(lib().lgh_csr_create(C.byref(h), device, code, len(self.indptr) - 1, self.indptr.ctypes.data,
+                                   self.indices.ctypes.data, vals.ctypes.data, len(self.indices)))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value and _lib is not None:
+            _lib.lgh_csr_destroy(h)
+            self.handle = None
+
+    def expval(self, state: State) -> complex:
+        re, im = C.c_double(), C.c_double()
+        check(lib().lgh_expval_csr(state.handle, self.handle, C.byref(re), C.byref(im)))
+        return complex(re.value, im.value)
+
+    def apply(self, out: State, x: State):
+        check(lib().lgh_apply_csr(out.handle, self.handle, x.handle))
+
+
+def adjoint_jacobian(psi: State, bras, ops, trainable, apply_operations=False) -> np.ndarray:
+    """Jacobian [obs][trainable]; ``bras[j]`` holds O_j psi on entry and is consumed."""
+    arr, keep = make_ops(ops)
+    hs = (C.c_void_p * max(1, len(bras)))(*[b.handle.value for b in bras])
+    tp = np.ascontiguousarray(list(trainable) or [0], dtype=np.int64)
+    jac = np.zeros(max(1, len(bras) * len(trainable)))
+    check(lib().lgh_adjoint_jacobian(psi.handle, hs, len(bras), arr, len(ops), tp.ctypes.data, len(trainable),
+                                     int(apply_operations), jac.ctypes.data))
+    return jac[: len(bras) * len(trainable)].reshape(len(bras), len(trainable))

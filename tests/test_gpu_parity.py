@@ -1,0 +1,334 @@
+"""GPU parity: the HIP path (through the C ABI and the pybind11 module) against
+the CPU oracle on the same seeded inputs.
+
+Tolerances: complex128 |d| <= 1e-10 * max(|ref|, 1e-3 * ||ref||_inf) (written
+as ``close128``); complex64 within 1e-5 of the complex128 oracle (``close64``);
+samples bit-exact when both sides sample the same amplitudes.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+N = pytest.importorskip("pennylane_lightning_gpu._native")
+
+
+def close128(got, ref, rel=1e-10):
+    got, ref = np.asarray(got), np.asarray(ref)
+    scale = np.maximum(np.abs(ref), 1e-3 * max(np.max(np.abs(ref)), 1e-300))
+    err = np.abs(got - ref)
+    assert np.all(err <= rel * scale + 1e-300), f"max rel err {np.max(err / scale):.3e}"
+
+
+def close64(got, ref):
+    got, ref = np.asarray(got), np.asarray(ref)
+    assert np.max(np.abs(got - ref)) <= 1e-5 * max(1.0, np.max(np.abs(ref))), \
+        f"max abs err {np.max(np.abs(got - ref)):.3e}"
+
+
+def rand_state(n, rng):
+    v = rng.normal(size=1 << n) + 1j * rng.normal(size=1 << n)
+    return v / np.linalg.norm(v)
+
+
+GATE_WIRES = {
+    "PauliX": 1, "PauliY": 1, "PauliZ": 1, "Hadamard": 1, "S": 1, "T": 1, "PhaseShift": 1, "RX": 1,
+    "RY": 1, "RZ": 1, "Rot": 1, "CNOT": 2, "SWAP": 2, "CY": 2, "CZ": 2, "ControlledPhaseShift": 2,
+    "CRX": 2, "CRY": 2, "CRZ": 2, "CRot": 2, "IsingXX": 2, "IsingYY": 2, "IsingZZ": 2,
+    "SingleExcitation": 2, "SingleExcitationMinus": 2, "SingleExcitationPlus": 2, "CSWAP": 3,
+    "Toffoli": 3, "DoubleExcitation": 4, "DoubleExcitationMinus": 4, "DoubleExcitationPlus": 4,
+    "MultiRZ": 3,
+}
+
+
+def params_for(name, rng):
+    return list(rng.uniform(-3, 3, N.PARAM_COUNT.get(name, 0)))
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("name", sorted(GATE_WIRES))
+def test_each_gate(name, dtype):
+    rng = np.random.default_rng(abs(hash(name)) % 2**32)
+    n = 9
+    psi = rand_state(n, rng)
+    nw = GATE_WIRES[name]
+    for trial in range(4):
+        # low, high and mixed target bits
+        wires = [int(w) for w in rng.choice(n, nw, replace=False)]
+        if trial == 0:
+            wires = list(range(n - nw, n))       # lowest amplitude bits
+        elif trial == 1:
+            wires = list(range(nw))              # highest amplitude bits
+        p = params_for(name, rng)
+        for inv in (False, True):
+            ops = [(name, wires, p, inv)]
+            ref = po.apply_ops(n, psi, ops)
+            s = N.State(n, dtype).upload(psi.astype(dtype))
+            got = s.apply(ops).download()
+            (close128 if dtype == np.complex128 else close64)(got, ref)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 12, 17])
+def test_random_circuits(n):
+    import test_oracle as T
+
+    rng = np.random.default_rng(500 + n)
+    ops = T.random_circuit(n, rng, depth=3) if n > 1 else [("RX", [0], [0.3], False), ("Hadamard", [0], [], False)]
+    psi = rand_state(n, rng)
+    ref = po.apply_ops(n, psi, ops)
+    close128(N.State(n).upload(psi).apply(ops).download(), ref)
+    close64(N.State(n, np.complex64).upload(psi.astype(np.complex64)).apply(ops).download(), ref)
+
+
+def test_matrix_gates_all_sizes():
+    rng = np.random.default_rng(8)
+    n = 10
+    psi = rand_state(n, rng)
+    for k in (1, 2, 3, 4, 5, 6):
+        A = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        Q, _ = np.linalg.qr(A)
+        wires = [int(w) for w in rng.choice(n, k, replace=False)]
+        for inv in (False, True):
+            ops = [("QubitUnitary", wires, [], inv, Q)]
+            close128(N.State(n).upload(psi).apply(ops).download(), po.apply_ops(n, psi, ops))
+
+
+def test_multirz_wide_and_basis_init():
+    rng = np.random.default_rng(9)
+    n = 12
+    psi = rand_state(n, rng)
+    ops = [("MultiRZ", [0, 2, 3, 5, 7, 8, 11], [0.77], False)]
+    close128(N.State(n).upload(psi).apply(ops).download(), po.apply_ops(n, psi, ops))
+    s = N.State(n).set_basis(37)
+    e = np.zeros(1 << n, complex)
+    e[37] = 1
+    assert np.array_equal(s.download(), e)
+    s.reset()
+    assert s.download()[0] == 1 and np.count_nonzero(s.download()) == 1
+    # setStateVector permutation (Test_StateVectorCudaManaged_NonParam.cpp:934-1014)
+    init = rand_state(3, rng)
+    idx = [0, 2, 4, 6, 1, 3, 5, 7]
+    vals = init[[1, 3, 5, 7, 0, 2, 4, 6]]
+    expected = init.copy()
+    for i in range(4):
+        expected[[2 * i, 2 * i + 1]] = expected[[2 * i + 1, 2 * i]]
+    np.testing.assert_array_equal(N.State(3).set_indexed(idx, vals).download(), expected)
+
+
+def test_generators_match_oracle():
+    rng = np.random.default_rng(10)
+    n = 6
+    psi = rand_state(n, rng)
+    for name in ("RX", "RY", "RZ", "PhaseShift", "CRX", "CRY", "CRZ", "ControlledPhaseShift", "IsingXX",
+                 "IsingYY", "IsingZZ", "SingleExcitation", "SingleExcitationMinus", "SingleExcitationPlus",
+                 "DoubleExcitation", "DoubleExcitationMinus", "DoubleExcitationPlus", "MultiRZ"):
+        nw = GATE_WIRES[name]
+        wires = [int(w) for w in rng.choice(n, nw, replace=False)]
+        s = N.State(n).upload(psi)
+        scale = s.apply_generator(name, wires)
+        close128(s.download(), po.apply_generator(n, psi, name, wires))
+        assert scale == po.generator(name, nw)[1]
+
+
+# ----------------------------------------------------------------------------- measurements
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_expval_matrix(dtype):
+    rng = np.random.default_rng(11)
+    n = 11
+    psi = rand_state(n, rng)
+    s = N.State(n, dtype).upload(psi.astype(dtype))
+    for k in (1, 2, 3, 4, 5, 7):
+        A = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+        H = A + A.conj().T
+        wires = [int(w) for w in rng.choice(n, k, replace=False)]
+        got = s.expval_matrix(wires, H)
+        ref = po.expval_matrix(n, psi, wires, H)
+        (close128 if dtype == np.complex128 else close64)(got.real, ref.real)
+
+
+def test_expval_literal_fixture(literals):
+    f = literals["dense_expval_8x8"]
+    psi = np.array([complex(*a) for a in f["state"]])
+    M = np.array([complex(*a) for a in f["matrix"]]).reshape(8, 8)
+    e = N.State(3).upload(psi).expval_matrix(f["wires"], M)
+    assert e.real == pytest.approx(f["expected"][0], rel=1e-7)
+    assert e.imag == pytest.approx(f["expected"][1], rel=1e-7)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_pauli_words(dtype):
+    rng = np.random.default_rng(12)
+    n = 12
+    psi = rand_state(n, rng)
+    words, wires, coeffs = [], [], []
+    for _ in range(40):
+        k = int(rng.integers(1, 5))
+        words.append("".join(rng.choice(list("XYZ"), k)))
+        wires.append([int(x) for x in rng.choice(n, k, replace=False)])
+        coeffs.append(rng.normal())
+    words += ["ZZ", "ZZ", "XX"]  # shared flip masks
+    wires += [[0, 11], [3, 4], [1, 10]]
+    coeffs += [0.5, -0.25, 1.5]
+    s = N.State(n, dtype).upload(psi.astype(dtype))
+    tot, per = s.expval_pauli_words(words, wires, coeffs)
+    rt, rp = po.expval_pauli_words(n, psi, words, wires, coeffs)
+    (close128 if dtype == np.complex128 else close64)(per, rp)
+    (close128 if dtype == np.complex128 else close64)(tot, rt)
+
+
+def tfim_csr(n, J=1.0, h=0.7):
+    import test_oracle as T
+
+    return T.tfim_csr(n, J, h)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_csr_expval_and_apply(literals, dtype):
+    f = literals["csr_expval"]
+    psi = np.array([complex(*a) for a in f["state"]])
+    data = np.array([complex(*a) for a in f["data"]])
+    c = N.CSR(f["indptr"], f["indices"], data, dtype=dtype)
+    assert c.expval(N.State(3, dtype).upload(psi.astype(dtype))).real == pytest.approx(1.0, rel=1e-6)
+    rng = np.random.default_rng(13)
+    for n in (6, 14):
+        H = tfim_csr(n)
+        psi = rand_state(n, rng)
+        s = N.State(n, dtype).upload(psi.astype(dtype))
+        c = N.CSR(H.indptr, H.indices, H.data, dtype=dtype)
+        ref = po.expval_csr(n, psi, H.indptr, H.indices.astype(np.int64), H.data)
+        (close128 if dtype == np.complex128 else close64)(c.expval(s).real, ref.real)
+        out = N.State(n, dtype)
+        c.apply(out, s)
+        (close128 if dtype == np.complex128 else close64)(out.download(), H @ psi)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_probs(dtype):
+    rng = np.random.default_rng(14)
+    n = 13
+    psi = rand_state(n, rng)
+    s = N.State(n, dtype).upload(psi.astype(dtype))
+    for wires in ([0], [12], [3, 7], [7, 3], [0, 1, 2, 3, 4, 5, 6, 7, 8, 9], list(range(n)),
+                  [12, 0, 5, 6, 1, 2, 3, 4, 8, 9, 10, 11]):
+        got = s.probs(wires)
+        ref = po.probs(n, psi, wires)
+        (close128 if dtype == np.complex128 else close64)(got, ref)
+
+
+@pytest.mark.parametrize("n", [3, 10, 16])
+def test_sampler_bit_exact(n):
+    rng = np.random.default_rng(15 + n)
+    psi = rand_state(n, rng)
+    for shots, seed in ((1, 1), (1000, 7), (100000, 1337)):
+        got = N.State(n).upload(psi).sample(shots, seed)
+        ref = po.sample(n, psi, shots, seed)
+        np.testing.assert_array_equal(got, ref)
+    p64 = psi.astype(np.complex64)
+    np.testing.assert_array_equal(N.State(n, np.complex64).upload(p64).sample(5000, 3), po.sample(n, p64, 5000, 3))
+    bits = N.State(n).upload(psi).sample_bits(257, 9)
+    idx = po.sample(n, psi, 257, 9)
+    expect = (idx[:, None] >> (n - 1 - np.arange(n))[None, :].astype(np.uint64)) & np.uint64(1)
+    np.testing.assert_array_equal(bits, expect)
+
+
+def test_inner_axpy_scale():
+    rng = np.random.default_rng(16)
+    n = 10
+    a, b = rand_state(n, rng), rand_state(n, rng)
+    A, B = N.State(n).upload(a), N.State(n).upload(b)
+    close128(A.inner(B), np.vdot(a, b))
+    A.axpy(0.3 - 0.2j, B)
+    close128(A.download(), a + (0.3 - 0.2j) * b)
+    B.scale(1j)
+    close128(B.download(), 1j * b)
+
+
+# ----------------------------------------------------------------------------- adjoint
+
+def layered_circuit(n, layers, rng, rot=("RX", "RY", "RZ"), ent="ring"):
+    ops = []
+    for _ in range(layers):
+        for w in range(n):
+            for g in rot:
+                ops.append((g, [w], [float(rng.uniform(0, 2 * math.pi))], False))
+        if ent == "ring":
+            for w in range(n):
+                ops.append(("CNOT", [w, (w + 1) % n], [], False))
+        else:  # CZ brick
+            for w in range(0, n - 1, 2):
+                ops.append(("CZ", [w, w + 1], [], False))
+            for w in range(1, n - 1, 2):
+                ops.append(("CZ", [w, w + 1], [], False))
+    return ops
+
+
+def obs_states(n, psi, obs_list, dtype=np.complex128):
+    out = []
+    for factors in obs_list:
+        v = psi
+        for name, wires in factors:
+            v = po.apply_ops(n, v, [(name, wires, [], False)])
+        out.append(v)
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_adjoint_config1_vs_oracle(dtype):
+    # 12 qubits, 4 layers RX,RY,RZ + CNOT ring, <Z_w> on all wires, 144 params
+    n = 12
+    rng = np.random.default_rng(1337)
+    ops = layered_circuit(n, 4, rng)
+    obs = [[("PauliZ", [w])] for w in range(n)]
+    psi = po.apply_ops(n, po.zero_state(n), ops)
+    bras = obs_states(n, psi, obs)
+    tp = list(range(4 * n * 3))
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    gpsi = N.State(n, dtype).upload(psi.astype(dtype))
+    gb = [N.State(n, dtype).upload(b.astype(dtype)) for b in bras]
+    got = N.adjoint_jacobian(gpsi, gb, ops, tp)
+    (close128 if dtype == np.complex128 else close64)(got, ref)
+    # forward state on the GPU from |0>
+    s = N.State(n, dtype).reset().apply(ops)
+    (close128 if dtype == np.complex128 else close64)(s.download(), psi)
+
+
+def test_adjoint_all_generators_vs_oracle():
+    n = 5
+    rng = np.random.default_rng(21)
+    init = rand_state(n, rng)
+    gates = [("PhaseShift", [1]), ("CRX", [0, 2]), ("CRY", [3, 1]), ("CRZ", [2, 0]),
+             ("ControlledPhaseShift", [1, 3]), ("IsingXX", [0, 4]), ("IsingYY", [2, 3]),
+             ("IsingZZ", [3, 0]), ("SingleExcitation", [1, 2]), ("SingleExcitationMinus", [0, 3]),
+             ("SingleExcitationPlus", [2, 1]), ("DoubleExcitation", [0, 1, 2, 3]),
+             ("DoubleExcitationMinus", [4, 2, 1, 0]), ("DoubleExcitationPlus", [1, 3, 0, 2]),
+             ("MultiRZ", [0, 2, 3]), ("RX", [2]), ("RY", [4]), ("RZ", [1])]
+    ops = []
+    for g, w in gates:
+        ops.append((g, w, [float(rng.uniform(-2, 2))], bool(rng.integers(2))))
+        ops.append(("Hadamard", [w[0]], [], False))
+        ops.append(("Toffoli", [w[0], (w[0] + 1) % n, (w[0] + 2) % n], [], False))
+    obs = [[("PauliX", [0]), ("PauliZ", [2])], [("PauliY", [3])], [("Hadamard", [4])]] * 4  # 12 bras: 2 batches
+    psi = po.apply_ops(n, init, ops)
+    bras = obs_states(n, psi, obs)
+    tp = [0, 2, 3, 5, 6, 7, 8, 10, 11, 12, 13, 14, 15, 16, 17]
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    got = N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp)
+    close128(got, ref)
+    # apply_operations=True starts from the initial state
+    got2 = N.adjoint_jacobian(N.State(n).upload(init), [N.State(n).upload(b) for b in bras], ops, tp,
+                              apply_operations=True)
+    close128(got2, ref)
+
+
+def test_adjoint_rejects_multiparam():
+    n = 2
+    s = N.State(n).reset()
+    b = N.State(n).reset()
+    with pytest.raises(N.LghError):
+        N.adjoint_jacobian(s, [b], [("Rot", [0], [0.1, 0.2, 0.3], False)], [0])
