@@ -48,9 +48,28 @@ def parse():
     ap.add_argument("--sweep-reps", type=int, default=10)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-ops", type=int, default=6)
+    ap.add_argument("--cpu-sample-ops", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1337)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run device-to-device copies of 1 GiB (PMC byte-counter calibration)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """1 GiB device copies: the known-size reference for the FETCH_SIZE /
+    WRITE_SIZE counters (read 1 GiB + write 1 GiB per copy)."""
+    from pennylane_lightning_gpu import _native as N
+
+    a, b = N.State(26).reset(), N.State(26)  # 2^26 * 16 B = 1 GiB each
+    for _ in range(3):
+        b.copy_from(a)
+    # the library's own streaming kernel of known size: y += a x reads 2 GiB, writes 1 GiB
+    for _ in range(3):
+        b.axpy(0.5, a)
+    b.synchronize()
+    print(json.dumps({"copy_calibration": {"copy": {"bytes_read": 1 << 30, "bytes_written": 1 << 30},
+                                           "k_axpy": {"bytes_read": 2 << 30, "bytes_written": 1 << 30},
+                                           "repeats": 3}}))
 
 
 class StreamTimer:
@@ -81,6 +100,9 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); every box in this setup has one GPU.",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import numpy as np
     import torch  # noqa: F401  (HIP events)
 
@@ -145,6 +167,11 @@ def main():
     fused_bytes = 4 * state_bytes
     fused_gbs = fused_bytes / (fused_ms * 1e-3) / 1e9
     del psi, bras
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if os.path.exists(tfile) and n == 28:
+        with open(tfile) as f:
+            traffic = json.load(f).get("k_adj_dense<double,1>@28q,1obs", {}).get("traffic_bytes")
 
     # ---------------------------------------------------------------- 30q single-gate sweep
     sweep = None
@@ -163,7 +190,12 @@ def main():
             ms = st.time(lambda: s.apply(op), reps=args.sweep_reps)
             rows.append({"target": t, "ms": round(ms, 4), "GBps": round(2 * sbytes / (ms * 1e-3) / 1e9, 1)})
         gbs = [r["GBps"] for r in rows]
-        sweep = {"qubits": sn, "gate": "RX", "bytes_per_pass": 2 * sbytes, "min_GBps": min(gbs),
+        # streaming ceiling of the same access (read + write every amplitude in place)
+        s.scale(1.0)
+        scale_ms = st.time(lambda: s.scale(1.0), reps=args.sweep_reps)
+        sweep = {"qubits": sn, "gate": "RX", "bytes_per_pass": 2 * sbytes,
+                 "inplace_scale_GBps": round(2 * sbytes / (scale_ms * 1e-3) / 1e9, 1),
+                 "min_GBps": min(gbs),
                  "median_GBps": float(np.median(gbs)), "max_GBps": max(gbs),
                  "min_frac_of_peak": round(min(gbs) / HBM_PEAK_GBS, 3), "per_target": rows}
         del s
@@ -204,7 +236,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "fused adjoint step (k_adj_dense<double,1>, 1 observable)",
                      "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(fused_gbs / HBM_PEAK_GBS, 3), "traffic": None,
+                     "frac": round(fused_gbs / HBM_PEAK_GBS, 3), "traffic": traffic,
                      "bytes_per_launch": fused_bytes, "ms_per_launch": round(fused_ms, 4)},
         "cpu_baseline": cpu,
         "gate_sweep": sweep,
