@@ -15,6 +15,7 @@
 //   (AdjointDiffGPU.hpp:132-162, :578-582; s from :96-114)
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 
 #include "lgh_internal.hpp"
 
@@ -57,10 +58,85 @@ template <class R> __device__ __forceinline__ double im_dot(cx<R> b, cx<double> 
     return double(b.re) * q.im - double(b.im) * q.re;
 }
 
-template <class R, int K>
+// Dense generator step for one 1-target gate with optional controls: one
+// amplitude pair per thread.  U^dagger in the state precision, G in double.
+template <class R, int NB>
+__global__ __launch_bounds__(kBlock) void k_adj_pair1(cx<R> *__restrict__ psi, cx<R> *const *__restrict__ bras, int nb,
+                                                      int upd_psi, uint64_t ngroups, Fixed fx, uint64_t cmask,
+                                                      uint64_t tbit, cx<R> u00, cx<R> u01, cx<R> u10, cx<R> u11,
+                                                      int with_gen, cx<double> g00, cx<double> g01, cx<double> g10,
+                                                      cx<double> g11, double *__restrict__ partials) {
+    double acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] = 0.0;
+    const uint64_t g = xcd_block() * kBlock + threadIdx.x;
+    if (g < ngroups) {
+        const uint64_t i0 = expand(g, fx) | cmask, i1 = i0 | tbit;
+        const cx<R> a0 = ldnt(psi + i0), a1 = ldnt(psi + i1);
+        const cx<double> q0 = cfma(g01, up(a1), cmul(g00, up(a0)));
+        const cx<double> q1 = cfma(g11, up(a1), cmul(g10, up(a0)));
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            if (j < nb) {
+                cx<R> *bj = bras[j];
+                const cx<R> b0 = ldnt(bj + i0), b1 = ldnt(bj + i1);
+                acc[j] += im_dot(b0, q0) + im_dot(b1, q1);
+                stnt(bj + i0, cfma(u01, b1, cmul(u00, b0)));
+                stnt(bj + i1, cfma(u11, b1, cmul(u10, b0)));
+            }
+        }
+        if (upd_psi) {
+            stnt(psi + i0, cfma(u01, a1, cmul(u00, a0)));
+            stnt(psi + i1, cfma(u11, a1, cmul(u10, a0)));
+        }
+    }
+    if (!with_gen) return;
+    block_sum<NB>(acc);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < nb; ++j) partials[size_t(j) * gridDim.x + blockIdx.x] = acc[j];
+}
+
+// Same step for an uncontrolled target bit t < 6: each thread streams one
+// amplitude of every state and takes its partner from lane ^ 2^t.
+template <class R, int NB>
+__global__ __launch_bounds__(kBlock) void k_adj_lane1(cx<R> *__restrict__ psi, cx<R> *const *__restrict__ bras, int nb,
+                                                      int upd_psi, uint64_t len, int tmask, cx<R> u00, cx<R> u01,
+                                                      cx<R> u10, cx<R> u11, int with_gen, cx<double> g00,
+                                                      cx<double> g01, cx<double> g10, cx<double> g11,
+                                                      double *__restrict__ partials) {
+    double acc[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) acc[j] = 0.0;
+    const uint64_t i = xcd_block() * kBlock + threadIdx.x;
+    const bool ok = i < len;
+    const bool hi = (threadIdx.x & tmask) != 0;
+    const cx<R> us = hi ? u11 : u00, up_ = hi ? u10 : u01;
+    const cx<double> gs = hi ? g11 : g00, gp = hi ? g10 : g01;
+    const cx<R> a = ok ? ldnt(psi + i) : cx<R>{R(0), R(0)};
+    const cx<R> ap = shfl_xor_cx(a, tmask);
+    const cx<double> q = cfma(gp, up(ap), cmul(gs, up(a)));
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (j < nb) {
+            cx<R> *bj = bras[j];
+            const cx<R> b = ok ? ldnt(bj + i) : cx<R>{R(0), R(0)};
+            const cx<R> bp = shfl_xor_cx(b, tmask);
+            acc[j] += im_dot(b, q);
+            if (ok) stnt(bj + i, cfma(up_, bp, cmul(us, b)));
+        }
+    }
+    if (upd_psi && ok) stnt(psi + i, cfma(up_, ap, cmul(us, a)));
+    if (!with_gen) return;
+    block_sum<NB>(acc);
+    if (threadIdx.x == 0)
+        for (int j = 0; j < nb; ++j) partials[size_t(j) * gridDim.x + blockIdx.x] = acc[j];
+}
+
+// K-target dense step (K = 2, 4): matrices staged in LDS, one group per thread.
+template <class R, int K, int NB>
 __global__ __launch_bounds__(kBlock) void k_adj_dense(cx<R> *__restrict__ psi, cx<R> *const *__restrict__ bras, int nb,
-                                                      int upd_psi, uint64_t ngroups, uint64_t chunk, Fixed fx,
-                                                      uint64_t cmask, const uint64_t *__restrict__ offs_g,
+                                                      int upd_psi, uint64_t ngroups, Fixed fx, uint64_t cmask,
+                                                      const uint64_t *__restrict__ offs_g,
                                                       const cx<R> *__restrict__ udag_g,
                                                       const cx<double> *__restrict__ gen_g,
                                                       double *__restrict__ partials) {
@@ -75,15 +151,15 @@ __global__ __launch_bounds__(kBlock) void k_adj_dense(cx<R> *__restrict__ psi, c
     for (int t = threadIdx.x; t < D; t += kBlock) O[t] = offs_g[t];
     __syncthreads();
     const bool with_gen = gen_g != nullptr;
-    double acc[kBras];
+    double acc[NB];
 #pragma unroll
-    for (int j = 0; j < kBras; ++j) acc[j] = 0.0;
-    const uint64_t lo = uint64_t(blockIdx.x) * chunk, hi = min(lo + chunk, ngroups);
-    for (uint64_t g = lo + threadIdx.x; g < hi; g += kBlock) {
+    for (int j = 0; j < NB; ++j) acc[j] = 0.0;
+    const uint64_t g = xcd_block() * kBlock + threadIdx.x;
+    if (g < ngroups) {
         const uint64_t base = expand(g, fx) | cmask;
         cx<R> a[D];
 #pragma unroll
-        for (int m = 0; m < D; ++m) a[m] = psi[base | O[m]];
+        for (int m = 0; m < D; ++m) a[m] = ldnt(psi + (base | O[m]));
         cx<double> q[D];
         if (with_gen) {
 #pragma unroll
@@ -95,12 +171,12 @@ __global__ __launch_bounds__(kBlock) void k_adj_dense(cx<R> *__restrict__ psi, c
             }
         }
 #pragma unroll
-        for (int j = 0; j < kBras; ++j) {
+        for (int j = 0; j < NB; ++j) {
             if (j < nb) {
                 cx<R> *bj = bras[j];
                 cx<R> b[D];
 #pragma unroll
-                for (int m = 0; m < D; ++m) b[m] = bj[base | O[m]];
+                for (int m = 0; m < D; ++m) b[m] = ldnt(bj + (base | O[m]));
                 if (with_gen) {
 #pragma unroll
                     for (int m = 0; m < D; ++m) acc[j] += im_dot(b[m], q[m]);
@@ -110,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_dense(cx<R> *__restrict__ psi, c
                     cx<R> s{R(0), R(0)};
 #pragma unroll
                     for (int c = 0; c < D; ++c) s = cfma(U[r * D + c], b[c], s);
-                    bj[base | O[r]] = s;
+                    stnt(bj + (base | O[r]), s);
                 }
             }
         }
@@ -120,105 +196,164 @@ __global__ __launch_bounds__(kBlock) void k_adj_dense(cx<R> *__restrict__ psi, c
                 cx<R> s{R(0), R(0)};
 #pragma unroll
                 for (int c = 0; c < D; ++c) s = cfma(U[r * D + c], a[c], s);
-                psi[base | O[r]] = s;
+                stnt(psi + (base | O[r]), s);
             }
         }
     }
     if (!with_gen) return;
-    block_sum<kBras>(acc);
+    block_sum<NB>(acc);
     if (threadIdx.x == 0)
         for (int j = 0; j < nb; ++j) partials[size_t(j) * gridDim.x + blockIdx.x] = acc[j];
 }
 
-template <class R>
+template <class R, int NB>
 __global__ __launch_bounds__(kBlock) void k_adj_diag(cx<R> *__restrict__ psi, cx<R> *const *__restrict__ bras, int nb,
-                                                     int upd_psi, uint64_t count, uint64_t chunk, Fixed fx,
-                                                     uint64_t cmask, uint64_t pmask, cx<R> ue, cx<R> uo, int with_gen,
-                                                     double ge, double go, double *__restrict__ partials) {
-    double acc[kBras];
+                                                     int upd_psi, uint64_t count, Fixed fx, uint64_t cmask,
+                                                     uint64_t pmask, cx<R> ue, cx<R> uo, int with_gen, double ge,
+                                                     double go, double *__restrict__ partials) {
+    double acc[NB];
 #pragma unroll
-    for (int j = 0; j < kBras; ++j) acc[j] = 0.0;
-    const uint64_t lo = uint64_t(blockIdx.x) * chunk, hi = min(lo + chunk, count);
-    for (uint64_t g = lo + threadIdx.x; g < hi; g += kBlock) {
+    for (int j = 0; j < NB; ++j) acc[j] = 0.0;
+    const uint64_t g = xcd_block() * kBlock + threadIdx.x;
+    if (g < count) {
         const uint64_t i = expand(g, fx) | cmask;
         const bool odd = __popcll(i & pmask) & 1;
-        const cx<R> a = psi[i];
+        const cx<R> a = ldnt(psi + i);
         const cx<R> u = odd ? uo : ue;
         const double gv = odd ? go : ge;
         const cx<double> q{gv * double(a.re), gv * double(a.im)};
 #pragma unroll
-        for (int j = 0; j < kBras; ++j) {
+        for (int j = 0; j < NB; ++j) {
             if (j < nb) {
-                const cx<R> b = bras[j][i];
+                const cx<R> b = ldnt(bras[j] + i);
                 acc[j] += im_dot(b, q);
-                bras[j][i] = cmul(u, b);
+                stnt(bras[j] + i, cmul(u, b));
             }
         }
-        if (upd_psi) psi[i] = cmul(u, a);
+        if (upd_psi) stnt(psi + i, cmul(u, a));
     }
     if (!with_gen) return;
-    block_sum<kBras>(acc);
+    block_sum<NB>(acc);
     if (threadIdx.x == 0)
         for (int j = 0; j < nb; ++j) partials[size_t(j) * gridDim.x + blockIdx.x] = acc[j];
 }
 
-// jac[j * ntp + col] = factor * sum_b partials[j][b]; one workgroup per j.
-__global__ __launch_bounds__(kBlock) void k_jac_store(const double *__restrict__ partials, int nparts,
-                                                      double *__restrict__ jac, int64_t ntp, int64_t col,
-                                                      double factor) {
-    const double *p = partials + size_t(blockIdx.x) * nparts;
+// Row sums of the per-workgroup partials in two deterministic stages:
+// k_row_chunks sums fixed contiguous slices of each row (grid: slices x rows),
+// k_jac_store sums the slice results of row j and scales them into the
+// Jacobian column.
+constexpr int kSlices = 256;
+
+__global__ __launch_bounds__(kBlock) void k_row_chunks(const double *__restrict__ partials, int nparts,
+                                                       double *__restrict__ slices) {
+    const int row = blockIdx.y;
+    const int per = (nparts + kSlices - 1) / kSlices;
+    const int lo = blockIdx.x * per, hi = min(lo + per, nparts);
+    const double *p = partials + size_t(row) * nparts;
     double v[1] = {0.0};
-    for (int b = threadIdx.x; b < nparts; b += kBlock) v[0] += p[b];
+    for (int b = lo + threadIdx.x; b < hi; b += kBlock) v[0] += p[b];
+    block_sum<1>(v);
+    if (threadIdx.x == 0) slices[size_t(row) * kSlices + blockIdx.x] = v[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_jac_store(const double *__restrict__ slices, double *__restrict__ jac,
+                                                      int64_t ntp, int64_t col, double factor) {
+    double v[1] = {threadIdx.x < kSlices ? slices[size_t(blockIdx.x) * kSlices + threadIdx.x] : 0.0};
     block_sum<1>(v);
     if (threadIdx.x == 0) jac[blockIdx.x * ntp + col] = factor * v[0];
 }
 
-struct Chunks {
-    uint64_t chunk;
-    unsigned blocks;
-};
-Chunks chunks_for(uint64_t count, int device) {
-    const uint64_t max_blocks = uint64_t(std::max(1, sm_count(device))) * 8;
-    uint64_t blocks = (count + kBlock * 4 - 1) / (kBlock * 4);
-    blocks = std::max<uint64_t>(1, std::min(blocks, max_blocks));
-    const uint64_t chunk = (count + blocks - 1) / blocks;
-    blocks = (count + chunk - 1) / chunk;
-    return {chunk, unsigned(std::max<uint64_t>(1, blocks))};
-}
-
 bool is_prep(const lgh_op &op) { return op.gate < 0; }
 
+// Launch geometry of one fused step: workgroups of kBlock threads, one
+// amplitude (lane kernel), pair or group per thread.
+struct StepShape {
+    enum Kind { LANE1, PAIR1, DENSE, DIAG } kind;
+    Fixed fx;
+    uint64_t count;
+    unsigned blocks;
+};
+
+StepShape step_shape(int n, const KOp &udag) {
+    StepShape s{};
+    if (udag.kind == KOp::DIAG) {
+        s.kind = StepShape::DIAG;
+        s.fx = make_fixed(udag.cbits);
+    } else {
+        std::vector<int> fixed_bits = udag.cbits;
+        for (int i = 0; i < udag.k; ++i) fixed_bits.push_back(udag.tbits[i]);
+        s.fx = make_fixed(fixed_bits);
+        if (udag.k == 1)
+            s.kind = (udag.cmask == 0 && udag.tbits[0] < 6 && n >= 6) ? StepShape::LANE1 : StepShape::PAIR1;
+        else
+            s.kind = StepShape::DENSE;
+    }
+    s.count = s.kind == StepShape::LANE1 ? (1ull << n) : (1ull << (n - s.fx.cnt));
+    s.blocks = unsigned((s.count + kBlock - 1) / kBlock);
+    return s;
+}
+
+// Calls f(std::integral_constant<int, NB>) with the smallest NB in {1, 2, 4, 8} >= nb.
+template <class F> void with_nb(int nb, F &&f) {
+    if (nb <= 1)
+        f(std::integral_constant<int, 1>{});
+    else if (nb <= 2)
+        f(std::integral_constant<int, 2>{});
+    else if (nb <= 4)
+        f(std::integral_constant<int, 4>{});
+    else
+        f(std::integral_constant<int, 8>{});
+}
+
 template <class R>
-void fused_step(lgh_sv *lam, cx<R> **bras_dev, int nobs, const KOp &udag, const KGen &gen, double *partials,
-                unsigned max_blocks_hint) {
-    (void)max_blocks_hint;
+void fused_step(lgh_sv *lam, cx<R> **bras_dev, int nobs, const KOp &udag, const KGen *gen, double *partials) {
     StreamCtx *ctx = lam->ctx;
     hipStream_t st = ctx->stream;
-    const int n = lam->n;
     cx<R> *psi = static_cast<cx<R> *>(lam->data);
     const int nbatches = std::max(1, (nobs + kBras - 1) / kBras);
-    if (udag.kind == KOp::DIAG) {
-        Fixed fx = make_fixed(udag.cbits);
-        const uint64_t count = 1ull << (n - fx.cnt);
-        Chunks ch = chunks_for(count, lam->device);
-        for (int b = 0; b < nbatches; ++b) {
-            const int nb = std::min(kBras, nobs - b * kBras);
-            hipLaunchKernelGGL(k_adj_diag<R>, dim3(ch.blocks), dim3(kBlock), 0, st, psi, bras_dev + b * kBras,
-                               std::max(nb, 0), int(b == nbatches - 1), count, ch.chunk, fx, udag.cmask, udag.pmask,
-                               to_cx<R>(udag.d_even), to_cx<R>(udag.d_odd), 1, gen.g_even, gen.g_odd,
-                               partials + size_t(b) * kBras * ch.blocks);
-            LGH_LAUNCH_CHECK();
-        }
+    const StepShape sh = step_shape(lam->n, udag);
+    const int wg = gen ? 1 : 0;
+    auto batch = [&](int b, auto launch) {
+        const int nb = std::max(0, std::min(kBras, nobs - b * kBras));
+        launch(bras_dev + b * kBras, nb, int(b == nbatches - 1), partials + size_t(b) * kBras * sh.blocks);
+        LGH_LAUNCH_CHECK();
+    };
+    if (sh.kind == StepShape::DIAG) {
+        const double ge = gen ? gen->g_even : 0.0, go = gen ? gen->g_odd : 0.0;
+        for (int b = 0; b < nbatches; ++b)
+            batch(b, [&](cx<R> **bp, int nb, int upd, double *pb) {
+                with_nb(nb, [&](auto NBc) {
+                    hipLaunchKernelGGL((k_adj_diag<R, decltype(NBc)::value>), dim3(sh.blocks), dim3(kBlock), 0, st,
+                                       psi, bp, nb, upd, sh.count, sh.fx, udag.cmask, udag.pmask,
+                                       to_cx<R>(udag.d_even), to_cx<R>(udag.d_odd), wg, ge, go, pb);
+                });
+            });
         return;
     }
-    // DENSE with K in {1, 2, 4}
-    std::vector<int> fixed_bits = udag.cbits;
-    for (int i = 0; i < udag.k; ++i) fixed_bits.push_back(udag.tbits[i]);
-    Fixed fx = make_fixed(fixed_bits);
-    const uint64_t ngroups = 1ull << (n - fx.cnt);
-    Chunks ch = chunks_for(ngroups, lam->device);
+    if (sh.kind != StepShape::DENSE) {
+        const cx<R> u00 = to_cx<R>(udag.mat[0]), u01 = to_cx<R>(udag.mat[1]), u10 = to_cx<R>(udag.mat[2]),
+                    u11 = to_cx<R>(udag.mat[3]);
+        cx<double> gm[4] = {};
+        if (gen)
+            for (int i = 0; i < 4; ++i) gm[i] = to_cx<double>(gen->mat[size_t(i)]);
+        for (int b = 0; b < nbatches; ++b)
+            batch(b, [&](cx<R> **bp, int nb, int upd, double *pb) {
+                with_nb(nb, [&](auto NBc) {
+                    constexpr int NB = decltype(NBc)::value;
+                    if (sh.kind == StepShape::LANE1)
+                        hipLaunchKernelGGL((k_adj_lane1<R, NB>), dim3(sh.blocks), dim3(kBlock), 0, st, psi, bp, nb,
+                                           upd, sh.count, 1 << udag.tbits[0], u00, u01, u10, u11, wg, gm[0], gm[1],
+                                           gm[2], gm[3], pb);
+                    else
+                        hipLaunchKernelGGL((k_adj_pair1<R, NB>), dim3(sh.blocks), dim3(kBlock), 0, st, psi, bp, nb,
+                                           upd, sh.count, sh.fx, udag.cmask, 1ull << udag.tbits[0], u00, u01, u10,
+                                           u11, wg, gm[0], gm[1], gm[2], gm[3], pb);
+                });
+            });
+        return;
+    }
+    // DENSE with K in {2, 4}: one upload of offsets | U^dagger | G
     const int D = 1 << udag.k;
-    // one upload: offsets | U^dagger (state precision) | G (double)
     const size_t b_off = sizeof(uint64_t) * D, b_u = sizeof(cx<R>) * D * D, b_g = sizeof(cx<double>) * D * D;
     const size_t o_u = (b_off + 15) & ~size_t(15), o_g = (o_u + b_u + 15) & ~size_t(15);
     std::vector<unsigned char> blob(o_g + b_g);
@@ -230,49 +365,30 @@ void fused_step(lgh_sv *lam, cx<R> **bras_dev, int nobs, const KOp &udag, const 
         offs[m] = o;
     }
     cx<R> *u = reinterpret_cast<cx<R> *>(blob.data() + o_u);
-    cx<double> *gm = reinterpret_cast<cx<double> *>(blob.data() + o_g);
+    cx<double> *gmat = reinterpret_cast<cx<double> *>(blob.data() + o_g);
     for (int i = 0; i < D * D; ++i) {
         u[i] = to_cx<R>(udag.mat[size_t(i)]);
-        gm[i] = to_cx<double>(gen.mat[size_t(i)]);
+        gmat[i] = gen ? to_cx<double>(gen->mat[size_t(i)]) : cx<double>{0.0, 0.0};
     }
     unsigned char *dblob = static_cast<unsigned char *>(ctx->upload(blob.data(), blob.size()));
     const uint64_t *d_off = reinterpret_cast<const uint64_t *>(dblob);
     const cx<R> *d_u = reinterpret_cast<const cx<R> *>(dblob + o_u);
-    const cx<double> *d_g = reinterpret_cast<const cx<double> *>(dblob + o_g);
-    for (int b = 0; b < nbatches; ++b) {
-        const int nb = std::max(0, std::min(kBras, nobs - b * kBras));
-        double *pb = partials + size_t(b) * kBras * ch.blocks;
-        const int upd = int(b == nbatches - 1);
-        switch (udag.k) {
-        case 1:
-            hipLaunchKernelGGL((k_adj_dense<R, 1>), dim3(ch.blocks), dim3(kBlock), 0, st, psi, bras_dev + b * kBras, nb,
-                               upd, ngroups, ch.chunk, fx, udag.cmask, d_off, d_u, d_g, pb);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_adj_dense<R, 2>), dim3(ch.blocks), dim3(kBlock), 0, st, psi, bras_dev + b * kBras, nb,
-                               upd, ngroups, ch.chunk, fx, udag.cmask, d_off, d_u, d_g, pb);
-            break;
-        case 4:
-            hipLaunchKernelGGL((k_adj_dense<R, 4>), dim3(ch.blocks), dim3(kBlock), 0, st, psi, bras_dev + b * kBras, nb,
-                               upd, ngroups, ch.chunk, fx, udag.cmask, d_off, d_u, d_g, pb);
-            break;
-        default:
-            throw Error(LGH_ERR_UNSUPPORTED, "no fused adjoint kernel for a " + std::to_string(udag.k) +
-                                                 "-target generator");
-        }
-        LGH_LAUNCH_CHECK();
-    }
-}
-
-// Partials of a fused step live in rows of kBras * blocks per batch; this
-// returns the number of workgroups the step uses so the store kernel knows
-// the row length.
-unsigned step_blocks(const lgh_sv *lam, const KOp &udag) {
-    std::vector<int> fixed_bits = udag.cbits;
-    if (udag.kind == KOp::DENSE)
-        for (int i = 0; i < udag.k; ++i) fixed_bits.push_back(udag.tbits[i]);
-    const uint64_t count = 1ull << (lam->n - int(fixed_bits.size()));
-    return chunks_for(count, lam->device).blocks;
+    const cx<double> *d_g = gen ? reinterpret_cast<const cx<double> *>(dblob + o_g) : nullptr;
+    for (int b = 0; b < nbatches; ++b)
+        batch(b, [&](cx<R> **bp, int nb, int upd, double *pb) {
+            if (udag.k != 2 && udag.k != 4)
+                throw Error(LGH_ERR_UNSUPPORTED, "no fused adjoint kernel for a " + std::to_string(udag.k) +
+                                                     "-target generator");
+            with_nb(nb, [&](auto NBc) {
+                constexpr int NB = decltype(NBc)::value;
+                if (udag.k == 2)
+                    hipLaunchKernelGGL((k_adj_dense<R, 2, NB>), dim3(sh.blocks), dim3(kBlock), 0, st, psi, bp, nb,
+                                       upd, sh.count, sh.fx, udag.cmask, d_off, d_u, d_g, pb);
+                else
+                    hipLaunchKernelGGL((k_adj_dense<R, 4, NB>), dim3(sh.blocks), dim3(kBlock), 0, st, psi, bp, nb,
+                                       upd, sh.count, sh.fx, udag.cmask, d_off, d_u, d_g, pb);
+            });
+        });
 }
 
 template <class R>
@@ -298,13 +414,16 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
             if (!is_prep(ops[i])) launch_kop(&lam, compile_op(ops[i], n, false));
 
     // workspace: [jac (nobs*ntp) | bra pointers | partials]
-    const unsigned max_blocks = unsigned(std::max(1, sm_count(psi_in->device)) * 8);
+    // the widest step shape has one thread per amplitude
+    const unsigned max_blocks = unsigned(((1ull << n) + kBlock - 1) / kBlock);
     const int nbatches = std::max(1, (nobs + kBras - 1) / kBras);
     auto al = [](size_t b) { return (b + 255) & ~size_t(255); };
     const size_t b_jac = al(sizeof(double) * size_t(std::max<int64_t>(1, nobs * ntp)));
     const size_t b_ptr = al(sizeof(void *) * size_t(nbatches * kBras));
     const size_t b_part = al(sizeof(double) * size_t(nbatches) * kBras * max_blocks);
-    char *ws = static_cast<char *>(ctx->workspace(b_jac + b_ptr + b_part));
+    const size_t b_slices = al(sizeof(double) * kBras * kSlices);
+    char *ws = static_cast<char *>(ctx->workspace(b_jac + b_ptr + b_part + b_slices));
+    double *slices = reinterpret_cast<double *>(ws + b_jac + b_ptr + b_part);
     double *jac_dev = reinterpret_cast<double *>(ws);
     cx<R> **bras_dev = reinterpret_cast<cx<R> **>(ws + b_jac);
     double *partials = reinterpret_cast<double *>(ws + b_jac + b_ptr);
@@ -329,18 +448,22 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
             const KGen gen = compile_generator(o, n, udag);
             if (udag.kind == KOp::DENSE && udag.k != 1 && udag.k != 2 && udag.k != 4)
                 throw Error(LGH_ERR_UNSUPPORTED, "no fused adjoint kernel for this operation");
-            fused_step<R>(&lam, bras_dev, nobs, udag, gen, partials, max_blocks);
-            const unsigned blocks = step_blocks(&lam, udag);
+            fused_step<R>(&lam, bras_dev, nobs, udag, &gen, partials);
+            const unsigned blocks = step_shape(n, udag).blocks;
             const double factor = -2.0 * gen.scale * (o.inverse ? -1.0 : 1.0);
             for (int b = 0; b < nbatches; ++b) {
                 const int nb = std::min(kBras, nobs - b * kBras);
                 if (nb <= 0) break;
-                hipLaunchKernelGGL(k_jac_store, dim3(nb), dim3(kBlock), 0, ctx->stream,
-                                   partials + size_t(b) * kBras * blocks, int(blocks),
+                hipLaunchKernelGGL(k_row_chunks, dim3(kSlices, nb), dim3(kBlock), 0, ctx->stream,
+                                   partials + size_t(b) * kBras * blocks, int(blocks), slices);
+                hipLaunchKernelGGL(k_jac_store, dim3(nb), dim3(kBlock), 0, ctx->stream, slices,
                                    jac_dev + size_t(b) * kBras * ntp, ntp, tp_pos, factor);
                 LGH_LAUNCH_CHECK();
             }
-        } else {
+        } else if (udag.kind == KOp::DIAG ||
+                   (udag.kind == KOp::DENSE && (udag.k == 1 || udag.k == 2 || udag.k == 4))) {
+            fused_step<R>(&lam, bras_dev, nobs, udag, nullptr, partials); // U^dagger on every state, one pass
+        } else if (udag.kind != KOp::NOP) {
             launch_kop(&lam, udag);
             for (int j = 0; j < nobs; ++j) launch_kop(bras[j], udag);
         }

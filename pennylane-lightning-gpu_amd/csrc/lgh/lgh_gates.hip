@@ -314,33 +314,33 @@ namespace {
 
 constexpr int kBlock = 256;
 
-// 1-target dense gate with optional controls.  Each thread streams E
-// amplitude pairs; consecutive lanes touch consecutive pair indices, so for
-// a target bit >= 6 every load of a wavefront is a contiguous 1 KiB (c128).
-template <class R, int E>
-__global__ __launch_bounds__(kBlock) void k_dense1(cx<R> *__restrict__ sv, uint64_t ngroups, Fixed fx,
-                                                   uint64_t cmask, uint64_t tbit, cx<R> m00, cx<R> m01,
-                                                   cx<R> m10, cx<R> m11) {
-    const uint64_t base = uint64_t(blockIdx.x) * (kBlock * E) + threadIdx.x;
-    uint64_t i0[E];
-    cx<R> a[E], b[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint64_t g = base + uint64_t(e) * kBlock;
-        if (g < ngroups) {
-            i0[e] = expand(g, fx) | cmask;
-            a[e] = sv[i0[e]];
-            b[e] = sv[i0[e] | tbit];
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint64_t g = base + uint64_t(e) * kBlock;
-        if (g < ngroups) {
-            sv[i0[e]] = cfma(m01, b[e], cmul(m00, a[e]));
-            sv[i0[e] | tbit] = cfma(m11, b[e], cmul(m10, a[e]));
-        }
-    }
+// 1-target dense gate with optional controls: one amplitude pair per thread,
+// consecutive lanes on consecutive pair indices (for a target bit >= 6 each
+// wavefront access is a contiguous 1 KiB for complex128).
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_gate1_pair(cx<R> *__restrict__ sv, uint64_t ngroups, Fixed fx,
+                                                       uint64_t cmask, uint64_t tbit, cx<R> m00, cx<R> m01,
+                                                       cx<R> m10, cx<R> m11) {
+    const uint64_t g = xcd_block() * kBlock + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint64_t i0 = expand(g, fx) | cmask;
+    const cx<R> a = ldnt(sv + i0), b = ldnt(sv + (i0 | tbit));
+    stnt(sv + i0, cfma(m01, b, cmul(m00, a)));
+    stnt(sv + (i0 | tbit), cfma(m11, b, cmul(m10, a)));
+}
+
+// 1-target dense gate without controls on target bit t < 6: the partner of
+// amplitude i sits in lane (lane ^ 2^t) of the same wavefront, so each thread
+// streams one amplitude and fetches its partner with a cross-lane shuffle.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_gate1_lane(cx<R> *__restrict__ sv, uint64_t len, int tmask, cx<R> m00,
+                                                       cx<R> m01, cx<R> m10, cx<R> m11) {
+    const uint64_t i = xcd_block() * kBlock + threadIdx.x;
+    const bool hi = (threadIdx.x & tmask) != 0;
+    const cx<R> cs = hi ? m11 : m00, cp = hi ? m10 : m01;
+    const cx<R> v = i < len ? ldnt(sv + i) : cx<R>{R(0), R(0)};
+    const cx<R> p = shfl_xor_cx(v, tmask);
+    if (i < len) stnt(sv + i, cfma(cp, p, cmul(cs, v)));
 }
 
 template <int K> struct Offs {
@@ -356,18 +356,18 @@ __global__ __launch_bounds__(kBlock) void k_denseK(cx<R> *__restrict__ sv, uint6
     __shared__ cx<R> sm[D * D];
     for (int t = threadIdx.x; t < D * D; t += kBlock) sm[t] = mat[t];
     __syncthreads();
-    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t g = xcd_block() * kBlock + threadIdx.x;
     if (g >= ngroups) return;
     const uint64_t base = expand(g, fx) | cmask;
     cx<R> v[D];
 #pragma unroll
-    for (int m = 0; m < D; ++m) v[m] = sv[base | offs.o[m]];
+    for (int m = 0; m < D; ++m) v[m] = ldnt(sv + (base | offs.o[m]));
 #pragma unroll
     for (int r = 0; r < D; ++r) {
         cx<R> acc{R(0), R(0)};
 #pragma unroll
         for (int c = 0; c < D; ++c) acc = cfma(sm[r * D + c], v[c], acc);
-        sv[base | offs.o[r]] = acc;
+        stnt(sv + (base | offs.o[r]), acc);
     }
 }
 
@@ -404,28 +404,14 @@ __global__ __launch_bounds__(kBlock) void k_denseLDS(cx<R> *__restrict__ sv, Fix
 
 // Diagonal phase: amplitudes with every cmask bit set are multiplied by
 // d_even / d_odd according to the parity of (index & pmask).
-template <class R, int E>
+template <class R>
 __global__ __launch_bounds__(kBlock) void k_diag(cx<R> *__restrict__ sv, uint64_t count, Fixed fx, uint64_t cmask,
                                                  uint64_t pmask, cx<R> de, cx<R> dodd) {
-    const uint64_t base = uint64_t(blockIdx.x) * (kBlock * E) + threadIdx.x;
-    uint64_t idx[E];
-    cx<R> v[E];
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint64_t g = base + uint64_t(e) * kBlock;
-        if (g < count) {
-            idx[e] = expand(g, fx) | cmask;
-            v[e] = sv[idx[e]];
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < E; ++e) {
-        const uint64_t g = base + uint64_t(e) * kBlock;
-        if (g < count) {
-            const bool odd = __popcll(idx[e] & pmask) & 1;
-            sv[idx[e]] = cmul(v[e], odd ? dodd : de);
-        }
-    }
+    const uint64_t g = xcd_block() * kBlock + threadIdx.x;
+    if (g >= count) return;
+    const uint64_t i = expand(g, fx) | cmask;
+    const bool odd = __popcll(i & pmask) & 1;
+    stnt(sv + i, cmul(ldnt(sv + i), odd ? dodd : de));
 }
 
 template <class R> void launch_kop_t(lgh_sv *sv, const KOp &op) {
@@ -437,9 +423,8 @@ template <class R> void launch_kop_t(lgh_sv *sv, const KOp &op) {
     if (op.kind == KOp::DIAG) {
         Fixed fx = make_fixed(op.cbits);
         const uint64_t count = 1ull << (n - fx.cnt);
-        constexpr int E = 4;
-        const uint64_t blocks = (count + kBlock * E - 1) / (kBlock * E);
-        hipLaunchKernelGGL((k_diag<R, E>), dim3(unsigned(blocks)), dim3(kBlock), 0, st, data, count, fx, op.cmask,
+        const uint64_t blocks = (count + kBlock - 1) / kBlock;
+        hipLaunchKernelGGL((k_diag<R>), dim3(unsigned(blocks)), dim3(kBlock), 0, st, data, count, fx, op.cmask,
                            op.pmask, to_cx<R>(op.d_even), to_cx<R>(op.d_odd));
         LGH_LAUNCH_CHECK();
         return;
@@ -450,11 +435,17 @@ template <class R> void launch_kop_t(lgh_sv *sv, const KOp &op) {
     const uint64_t ngroups = 1ull << (n - fx.cnt);
     const int D = 1 << op.k;
     if (op.k == 1) {
-        constexpr int E = 4;
-        const uint64_t blocks = (ngroups + kBlock * E - 1) / (kBlock * E);
-        hipLaunchKernelGGL((k_dense1<R, E>), dim3(unsigned(blocks)), dim3(kBlock), 0, st, data, ngroups, fx,
-                           op.cmask, 1ull << op.tbits[0], to_cx<R>(op.mat[0]), to_cx<R>(op.mat[1]),
-                           to_cx<R>(op.mat[2]), to_cx<R>(op.mat[3]));
+        const cx<R> m00 = to_cx<R>(op.mat[0]), m01 = to_cx<R>(op.mat[1]), m10 = to_cx<R>(op.mat[2]),
+                    m11 = to_cx<R>(op.mat[3]);
+        if (op.cmask == 0 && op.tbits[0] < 6 && n >= 6) {
+            const uint64_t len = 1ull << n;
+            hipLaunchKernelGGL((k_gate1_lane<R>), dim3(unsigned((len + kBlock - 1) / kBlock)), dim3(kBlock), 0, st,
+                               data, len, 1 << op.tbits[0], m00, m01, m10, m11);
+        } else {
+            const uint64_t blocks = (ngroups + kBlock - 1) / kBlock;
+            hipLaunchKernelGGL((k_gate1_pair<R>), dim3(unsigned(blocks)), dim3(kBlock), 0, st, data, ngroups, fx,
+                               op.cmask, 1ull << op.tbits[0], m00, m01, m10, m11);
+        }
         LGH_LAUNCH_CHECK();
         return;
     }

@@ -81,6 +81,35 @@ template <class R> cx<R> to_cx(std::complex<double> z) {
     return cx<R>{static_cast<R>(z.real()), static_cast<R>(z.imag())};
 }
 
+// Streaming accesses of whole amplitudes (one 16 B / 8 B vector access) with
+// the nontemporal hint: every state pass touches each amplitude once, so
+// keeping it out of the caches leaves them to the data that is re-used.
+template <class R> using vec2_t = R __attribute__((ext_vector_type(2)));
+
+template <class R> __device__ __forceinline__ cx<R> ldnt(const cx<R> *p) {
+    const vec2_t<R> v = __builtin_nontemporal_load(reinterpret_cast<const vec2_t<R> *>(p));
+    return cx<R>{v.x, v.y};
+}
+template <class R> __device__ __forceinline__ void stnt(cx<R> *p, cx<R> a) {
+    vec2_t<R> v;
+    v.x = a.re;
+    v.y = a.im;
+    __builtin_nontemporal_store(v, reinterpret_cast<vec2_t<R> *>(p));
+}
+
+// XCD-aware workgroup order: the hardware hands consecutive workgroups to the
+// 8 XCDs round-robin; remapping so that XCD k processes the k-th contiguous
+// eighth of the index space keeps each XCD's memory stream sequential.
+__device__ __forceinline__ uint64_t xcd_block() {
+    const uint64_t g = gridDim.x;
+    if (g % 8 != 0) return blockIdx.x;
+    return (blockIdx.x % 8) * (g / 8) + blockIdx.x / 8;
+}
+
+template <class R> __device__ __forceinline__ cx<R> shfl_xor_cx(cx<R> a, int mask) {
+    return cx<R>{__shfl_xor(a.re, mask, 64), __shfl_xor(a.im, mask, 64)};
+}
+
 constexpr int kMaxFixed = 16; // target + control bits that a kernel enumerates around
 constexpr int kMaxRegK = 4;   // dense gates held in registers up to 4 targets
 constexpr int kMaxLdsK = 13;  // dense gates staged through LDS up to 13 targets
