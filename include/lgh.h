@@ -123,6 +123,10 @@ lgh_status lgh_device_synchronize(int device);
 lgh_status lgh_device_reset(void);
 /* Returns cached device buffers of `device` to the runtime. */
 lgh_status lgh_pool_trim(int device);
+/* Gate fusion switch for lgh_apply sequences and the adjoint sweep (on by
+ * default).  No counterpart in the reference, which applies one kernel per
+ * gate; exposed for comparisons and tests. */
+lgh_status lgh_set_fusion(int enabled);
 /* Streams for callers that overlap independent states (serving). */
 lgh_status lgh_stream_create(int device, void **stream);
 lgh_status lgh_stream_destroy(void *stream);

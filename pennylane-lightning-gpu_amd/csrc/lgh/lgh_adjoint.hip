@@ -17,6 +17,7 @@
 #include <cstring>
 #include <type_traits>
 
+#include "lgh_fused.hpp"
 #include "lgh_internal.hpp"
 
 using namespace lgh;
@@ -422,8 +423,10 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
     const size_t b_ptr = al(sizeof(void *) * size_t(nbatches * kBras));
     const size_t b_part = al(sizeof(double) * size_t(nbatches) * kBras * max_blocks);
     const size_t b_slices = al(sizeof(double) * kBras * kSlices);
-    char *ws = static_cast<char *>(ctx->workspace(b_jac + b_ptr + b_part + b_slices));
+    const size_t b_fused = al(fused_partials_bytes(n, psi_in->device));
+    char *ws = static_cast<char *>(ctx->workspace(b_jac + b_ptr + b_part + b_slices + b_fused));
     double *slices = reinterpret_cast<double *>(ws + b_jac + b_ptr + b_part);
+    double *fpartials = reinterpret_cast<double *>(ws + b_jac + b_ptr + b_part + b_slices);
     double *jac_dev = reinterpret_cast<double *>(ws);
     cx<R> **bras_dev = reinterpret_cast<cx<R> **>(ws + b_jac);
     double *partials = reinterpret_cast<double *>(ws + b_jac + b_ptr);
@@ -432,6 +435,78 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
     LGH_HIP(hipMemcpyAsync(bras_dev, hp.data(), sizeof(void *) * hp.size(), hipMemcpyHostToDevice, ctx->stream));
     LGH_HIP(hipMemsetAsync(jac_dev, 0, sizeof(double) * size_t(std::max<int64_t>(1, nobs * ntp)), ctx->stream));
     ctx->sync(); // hp is a host temporary
+
+    if (fusion_enabled() && b_fused > 0 && nobs > 0) {
+        // Fused reverse sweep, one observable state at a time on a fresh copy
+        // of lambda.  Ops before the earliest trainable one are never visited.
+        std::vector<KOp> udag;
+        std::vector<KGen> gens;
+        std::vector<int> slot;
+        std::vector<double> fac;
+        int64_t pidx = -1, k = 0;
+        bool started = false;
+        for (int64_t i = 0; i < nops; ++i) {
+            const lgh_op &o = ops[i];
+            if (is_prep(o)) continue;
+            int col = -1;
+            if (o.num_params > 0) {
+                ++pidx;
+                if (k < ntp && pidx == tp[k]) col = int(k++);
+            }
+            if (col < 0 && !started) continue;
+            started = true;
+            udag.push_back(compile_op(o, n, true));
+            if (col >= 0) {
+                gens.push_back(compile_generator(o, n, udag.back()));
+                fac.push_back(-2.0 * gens.back().scale * (o.inverse ? -1.0 : 1.0));
+            } else {
+                gens.emplace_back();
+                fac.push_back(0.0);
+            }
+            slot.push_back(col);
+        }
+        lgh_sv lam0 = lam; // forward state every observable sweep starts from
+        lam0.data = nullptr;
+        if (nobs > 1) {
+            lam0.data = pool_alloc(psi_in->device, psi_in->bytes);
+            LGH_HIP(hipMemcpyAsync(lam0.data, lam.data, lam.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        struct Free0 {
+            lgh_sv *s;
+            ~Free0() {
+                if (!s->data) return;
+                (void)hipStreamSynchronize(s->ctx->stream);
+                pool_free(s->device, s->data, s->bytes);
+            }
+        } free_lam0{&lam0};
+        for (int j = 0; j < nobs; ++j) {
+            if (j > 0)
+                LGH_HIP(hipMemcpyAsync(lam.data, lam0.data, lam.bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            double *jrow = jac_dev + size_t(j) * ntp;
+            auto standalone = [&](const KOp &u, const KGen *g, int64_t col, double f) {
+                const bool fusedk = u.kind == KOp::DIAG || (u.kind == KOp::DENSE && (u.k == 1 || u.k == 2 || u.k == 4));
+                if (!fusedk) {
+                    launch_kop(&lam, u);
+                    launch_kop(bras[j], u);
+                    return;
+                }
+                fused_step<R>(&lam, bras_dev + j, 1, u, g, partials);
+                if (g) {
+                    const unsigned blocks = step_shape(n, u).blocks;
+                    hipLaunchKernelGGL(k_row_chunks, dim3(kSlices, 1), dim3(kBlock), 0, ctx->stream, partials,
+                                       int(blocks), slices);
+                    hipLaunchKernelGGL(k_jac_store, dim3(1), dim3(kBlock), 0, ctx->stream, slices, jrow, ntp, col, f);
+                    LGH_LAUNCH_CHECK();
+                }
+            };
+            fused_adjoint_one(&lam, bras[j], udag, gens, slot, fac, jrow, fpartials, standalone);
+        }
+        if (nobs * ntp > 0)
+            LGH_HIP(hipMemcpyAsync(jac_host, jac_dev, sizeof(double) * size_t(nobs * ntp), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        ctx->sync();
+        return;
+    }
 
     int64_t num_param_ops = 0;
     for (int64_t i = 0; i < nops; ++i)

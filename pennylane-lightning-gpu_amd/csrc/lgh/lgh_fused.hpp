@@ -1,0 +1,44 @@
+// Gate fusion (lgh_fused.hip): pass scheduling and fused forward / adjoint
+// passes.  Internal to liblgh.so.
+#pragma once
+
+#include <functional>
+#include <vector>
+
+#include "lgh_internal.hpp"
+
+namespace lgh {
+
+// One fused pass: gates (indices into the compiled op list, in execution
+// order) whose dense targets are lane bits (0..5) or one of the register
+// bits.  A standalone pass holds one gate that the fused kernels cannot run.
+struct Pass {
+    bool standalone = false;
+    int reg_bits[4] = {0, 0, 0, 0};
+    std::vector<int> ops;
+};
+
+void set_fusion(int enabled);
+int fusion_enabled();
+bool fusable(const KOp &k, int n);
+// trainable_slot[i] >= 0 marks gates whose generator overlap is accumulated
+// (at most 32 per pass); empty for the forward direction.
+std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot);
+int count_passes(const std::vector<KOp> &kops, int n);
+
+// Forward application of a compiled op list through fused passes.
+void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops);
+
+// Applies U^dagger of one non-fusable gate to lambda and the observable
+// state; with a generator, stores factor * Im<bra|G lambda> into jac_row[col].
+using StandaloneStep = std::function<void(const KOp &udag, const KGen *gen, int64_t col, double factor)>;
+
+// Reverse sweep of one observable state through fused passes.  udag: the
+// ops in forward order compiled as U^dagger; slot_col[i]: Jacobian column of
+// a trainable gate or -1; factor[i] = -2 s (inverse ? -1 : 1).
+void fused_adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, const std::vector<KGen> &gens,
+                       const std::vector<int> &slot_col, const std::vector<double> &factor, double *jac_row_dev,
+                       double *partials, StandaloneStep standalone);
+size_t fused_partials_bytes(int n, int device);
+
+} // namespace lgh

@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "lgh_fused.hpp"
 #include "lgh_internal.hpp"
 
 namespace lgh {
@@ -526,6 +527,13 @@ lgh_status lgh_apply(lgh_sv *sv, const lgh_op *ops, int64_t num_ops) {
         // validate everything before launching anything
         for (int64_t i = 0; i < num_ops; ++i) validate_op(ops[i], sv->n);
         DeviceGuard g(sv->device);
+        if (num_ops >= 2 && fusion_enabled()) {
+            std::vector<KOp> kops;
+            kops.reserve(size_t(num_ops));
+            for (int64_t i = 0; i < num_ops; ++i) kops.push_back(compile_op(ops[i], sv->n, false));
+            fused_apply(sv, kops);
+            return;
+        }
         for (int64_t i = 0; i < num_ops; ++i) launch_kop(sv, compile_op(ops[i], sv->n, false));
     });
 }

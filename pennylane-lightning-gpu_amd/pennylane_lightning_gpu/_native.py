@@ -112,6 +112,11 @@ def check(rc):
         raise LghError(rc, lib().lgh_last_error().decode())
 
 
+def set_fusion(enabled: bool):
+    """Switch gate fusion of lgh_apply sequences and the adjoint sweep."""
+    check(lib().lgh_set_fusion(int(bool(enabled))))
+
+
 def device_count() -> int:
     c = C.c_int()
     check(lib().lgh_device_count(C.byref(c)))

@@ -332,3 +332,80 @@ def test_adjoint_rejects_multiparam():
     b = N.State(n).reset()
     with pytest.raises(N.LghError):
         N.adjoint_jacobian(s, [b], [("Rot", [0], [0.1, 0.2, 0.3], False)], [0])
+
+
+# ----------------------------------------------------------------------------- gate fusion
+
+def mixed_circuit(n, rng, layers=3):
+    ops = []
+    one = ["RX", "RY", "RZ", "PhaseShift", "Hadamard", "S", "T", "PauliX", "PauliY", "PauliZ", "Rot"]
+    two = ["CNOT", "CZ", "CY", "CRX", "CRY", "CRZ", "ControlledPhaseShift", "IsingXX", "IsingYY", "IsingZZ",
+           "SWAP", "SingleExcitation", "CRot"]
+    for _ in range(layers):
+        for w in range(n):
+            g = one[rng.integers(len(one))]
+            ops.append((g, [w], list(rng.uniform(-3, 3, N.PARAM_COUNT.get(g, 0))), bool(rng.integers(2))))
+        for _ in range(n):
+            a, b = (int(x) for x in rng.choice(n, 2, replace=False))
+            g = two[rng.integers(len(two))]
+            ops.append((g, [a, b], list(rng.uniform(-3, 3, N.PARAM_COUNT.get(g, 0))), bool(rng.integers(2))))
+        a, b, c, d = (int(x) for x in rng.choice(n, 4, replace=False))
+        ops.append(("Toffoli", [a, b, c], [], False))
+        ops.append(("MultiRZ", [a, c, d], [float(rng.uniform(-3, 3))], False))
+        ops.append(("DoubleExcitation", [a, b, c, d], [float(rng.uniform(-3, 3))], False))
+        ops.append(("CSWAP", [d, a, b], [], False))
+    return ops
+
+
+@pytest.mark.parametrize("n", [10, 13, 16])
+def test_fused_forward_matches_oracle(n):
+    rng = np.random.default_rng(900 + n)
+    ops = mixed_circuit(n, rng)
+    psi = rand_state(n, rng)
+    ref = po.apply_ops(n, psi, ops)
+    try:
+        for fuse in (True, False):
+            N.set_fusion(fuse)
+            close128(N.State(n).upload(psi).apply(ops).download(), ref)
+            close64(N.State(n, np.complex64).upload(psi.astype(np.complex64)).apply(ops).download(), ref)
+    finally:
+        N.set_fusion(True)
+
+
+@pytest.mark.parametrize("n", [11, 14])
+def test_fused_adjoint_matches_oracle(n):
+    rng = np.random.default_rng(950 + n)
+    # differentiable gates carry at most one parameter
+    ops = [o for o in mixed_circuit(n, rng, layers=2) if len(o[2]) <= 1]
+    init = rand_state(n, rng)
+    psi = po.apply_ops(n, init, ops)
+    obs = [[("PauliZ", [0])], [("PauliX", [n - 1]), ("PauliY", [3])], [("Hadamard", [5])]]
+    bras = obs_states(n, psi, obs)
+    npar = sum(1 for o in ops if o[2])
+    tp = sorted(int(x) for x in rng.choice(npar, npar // 2, replace=False))
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    try:
+        for fuse in (True, False):
+            N.set_fusion(fuse)
+            got = N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp)
+            close128(got, ref)
+            got64 = N.adjoint_jacobian(N.State(n, np.complex64).upload(psi.astype(np.complex64)),
+                                       [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras],
+                                       ops, tp)
+            close64(got64, ref)
+    finally:
+        N.set_fusion(True)
+
+
+def test_fused_adjoint_config2_shape():
+    # RY/RZ + CZ brick (the 28-qubit benchmark circuit family) at 14 qubits
+    from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
+
+    n = 14
+    ops = ry_rz_cz_brick(n, 3, seed=5)
+    psi = po.apply_ops(n, po.zero_state(n), ops)
+    bra = po.apply_ops(n, psi, [("PauliZ", [0], [], False), ("PauliZ", [n - 1], [], False)])
+    tp = list(range(sum(1 for o in ops if o[2])))
+    ref = po.adjoint_jacobian(n, psi, [bra], ops, tp)
+    got = N.adjoint_jacobian(N.State(n).reset().apply(ops), [N.State(n).upload(bra)], ops, tp)
+    close128(got, ref)
