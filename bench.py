@@ -50,6 +50,17 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample-ops", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1337)
+    ap.add_argument("--tile-bits", type=int, nargs=2, default=None, metavar=("FWD", "ADJ"),
+                    help="register bits of the fused forward / adjoint passes (library default 4 3)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-arrivals", type=int, default=400)
+    ap.add_argument("--serve-rate", type=float, default=2000.0)
+    ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--batch-circuits", type=int, default=4096)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--batch-serial-sample", type=int, default=256,
+                    help="circuits timed one after another for the per-circuit comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run device-to-device copies of 1 GiB (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -94,6 +105,134 @@ class StreamTimer:
         return start.elapsed_time(end) / reps  # ms
 
 
+def bench_batched(N, args):
+    """Batched workload: circuits with n ~ U{8..20} qubits (seeded), each 3 x
+    [RX, RY, RZ on every wire; CNOT ring] (12n gates, 9n parameters), <Z_0>
+    and its gradient; complex128.  One step = all circuits in one persistent
+    launch.  Compared with the per-circuit launch form on a sample of the
+    same circuits."""
+    import numpy as np
+
+    from pennylane_lightning_gpu.workloads import batch_widths, ring_template
+
+    widths = batch_widths(args.batch_circuits, seed=args.seed)
+    rng = np.random.default_rng(args.seed)
+    tm = {}
+    for w in sorted(set(widths)):
+        ops, pidx = ring_template(w)
+        tm[w] = N.Template(w, ops, pidx, [("Z", 0)])
+    ts = [tm[w] for w in widths]
+    thetas = [rng.uniform(0, 2 * np.pi, t.num_params) for t in ts]
+    log(f"[bench] batched: {len(ts)} circuits, n in [{min(widths)}, {max(widths)}], forward + gradient")
+    N.batch_run(ts, thetas)  # warm-up
+    N.profile_enable(True)
+    steps = []
+    for i in range(args.batch_steps):
+        t0 = time.perf_counter()
+        ev, jac = N.batch_run(ts, thetas)
+        steps.append(time.perf_counter() - t0)
+        log(f"[bench] batched step {i}: {steps[-1] * 1e3:.1f} ms")
+    kn, kms = N.profile_read("batch")
+    N.profile_enable(False)
+    k = min(args.batch_serial_sample, len(ts))
+    t0 = time.perf_counter()
+    ev_s, jac_s = N.batch_run(ts[:k], thetas[:k], serial=True)
+    t_serial = time.perf_counter() - t0
+    diff = max(float(np.max(np.abs(ev[:k] - ev_s))),
+               max(float(np.max(np.abs(a - b))) for a, b in zip(jac[:k], jac_s)))
+    st = np.array(steps)
+    cps = len(ts) / float(np.median(st))
+    # the sample's share of the full batch's work, by circuit cost ~ n 2^n
+    cost = np.array([w * 2.0 ** w for w in widths])
+    serial_full = t_serial * cost.sum() / cost[:k].sum()
+    return {
+        "circuits": len(ts), "qubits": [min(widths), max(widths)], "dtype": "c128",
+        "circuits_per_s": round(cps, 1),
+        "step_ms_p50": round(float(np.percentile(st, 50)) * 1e3, 2),
+        "step_ms_p99": round(float(np.percentile(st, 99)) * 1e3, 2),
+        "kernel_ms_per_step": round(kms / max(1, kn), 2),
+        "per_circuit_form": {"sample": k, "seconds": round(t_serial, 3),
+                             "circuits_per_s_est": round(len(ts) / serial_full, 1),
+                             "max_abs_diff_vs_batched": diff},
+        "speedup_vs_per_circuit": round(serial_full / float(np.median(st)), 1),
+    }
+
+
+def bench_serving(N, args):
+    """Serving workload: Poisson arrivals (seeded exponential gaps) at
+    ``--serve-rate`` per second; n from {10, 14, 18, 22, 26, 30} with weights
+    8:6:4:3:2:1; layers ~ U{2..8} of [RY, RZ on every wire; CZ on even then odd
+    pairs]; complex128; half the requests ask for <Z_0>, half for its gradient.
+    A fixed number of arrivals is submitted on schedule and drained."""
+    import threading
+
+    import numpy as np
+
+    rng = np.random.default_rng(args.seed)
+    sizes = np.array([10, 14, 18, 22, 26, 30])
+    weights = np.array([8, 6, 4, 3, 2, 1], dtype=float)
+    reqs = []
+    t = 0.0
+    for i in range(args.serve_arrivals):
+        t += rng.exponential(1.0 / args.serve_rate)
+        n = int(rng.choice(sizes, p=weights / weights.sum()))
+        layers = int(rng.integers(2, 9))
+        ops = []
+        for _ in range(layers):
+            for w in range(n):
+                ops.append(("RY", [w], [float(rng.uniform(0, 2 * np.pi))], False))
+                ops.append(("RZ", [w], [float(rng.uniform(0, 2 * np.pi))], False))
+            for s0 in (0, 1):
+                for w in range(s0, n - 1, 2):
+                    ops.append(("CZ", [w, w + 1], [], False))
+        reqs.append((t, n, ops, bool(i % 2)))
+    srv = N.Server(num_streams=args.serve_streams)
+    # warm the template cache and the allocator with one request per class
+    for n in sizes:
+        srv.wait(srv.submit(int(n), [("RY", [0], [0.1], False)], [("Z", 0)], True))
+    log(f"[bench] serving: {len(reqs)} arrivals at {args.serve_rate}/s over {reqs[-1][0]:.2f} s")
+    tickets = [None] * len(reqs)
+    late = np.zeros(len(reqs))  # submission after the scheduled arrival (host side), ms
+    t0 = time.perf_counter()
+    for i, (ta, n, ops, grad) in enumerate(reqs):
+        dt = ta - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        late[i] = max(0.0, (time.perf_counter() - t0) - ta) * 1e3
+        tickets[i] = srv.submit(n, ops, [("Z", 0)], grad)
+    t_sub = time.perf_counter() - t0
+    lat = {}
+    for i, ((ta, n, ops, grad), tk) in enumerate(zip(reqs, tickets)):
+        _, _, tm = srv.wait(tk)
+        # queueing counted from the scheduled arrival
+        lat.setdefault(int(n), []).append((tm[0] + late[i], tm[1], tm[2] + late[i]))
+    t_all = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+
+    def pct(v, q):
+        return round(float(np.percentile(v, q)), 2)
+
+    per = {}
+    alltm = []
+    for n in sorted(lat):
+        a = np.array(lat[n])
+        alltm.append(a)
+        per[str(n)] = {"count": len(a), "latency_ms_p50": pct(a[:, 2], 50), "latency_ms_p99": pct(a[:, 2], 99),
+                       "queue_ms_p50": pct(a[:, 0], 50), "queue_ms_p99": pct(a[:, 0], 99),
+                       "run_ms_p50": pct(a[:, 1], 50)}
+    a = np.concatenate(alltm)
+    return {
+        "arrivals": len(reqs), "offered_rate": args.serve_rate, "streams": args.serve_streams,
+        "submit_window_s": round(t_sub, 3), "drain_s": round(t_all, 3),
+        "submit_lateness_ms_p99": pct(late, 99),
+        "achieved_circuits_per_s": round(len(reqs) / t_all, 1),
+        "latency_ms_p50": pct(a[:, 2], 50), "latency_ms_p99": pct(a[:, 2], 99),
+        "queue_ms_p50": pct(a[:, 0], 50), "queue_ms_p99": pct(a[:, 0], 99),
+        "per_class": per, "engine": st,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -110,6 +249,8 @@ def main():
     from pennylane_lightning_gpu import lightning_gpu_qubit_ops as ops_mod
     from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
 
+    if args.tile_bits:
+        N.set_tile_bits(*args.tile_bits)
     n, layers = args.qubits, args.layers
     ops = ry_rz_cz_brick(n, layers, seed=args.seed)
     n_par = sum(1 for o in ops if o[2])
@@ -134,6 +275,9 @@ def main():
     for _ in range(args.warmup):
         jac = step()
     N.check(N.lib().lgh_device_synchronize(0))
+    # HIP events around every fused-pass launch of the timed steps, recorded
+    # by the library on the stream it launches on
+    N.profile_enable(True)
     times = []
     for i in range(args.steps):
         N.check(N.lib().lgh_device_synchronize(0))
@@ -144,34 +288,39 @@ def main():
         log(f"[bench] step {i}: {times[-1]:.3f} s")
     sec = float(np.mean(times))
     jac = np.asarray(jac).ravel()
+    adj_n, adj_ms = N.profile_read("fused_adjoint")
+    fwd_n, fwd_ms = N.profile_read("fused_pass")
+    N.profile_enable(False)
 
-    # ---------------------------------------------------------------- dominant kernel: fused adjoint step
-    # Per fused step with one observable state: read+write lambda and the
-    # observable state = 4 * state bytes.  Measured as the time difference of
-    # two adjoint calls with 2K and K trainable RY ops (fixed costs cancel).
-    psi = N.State(n).reset()
-    ry_ops = [("RY", [w % n], [0.1 + 0.01 * w], False) for w in range(2 * n)]
-    psi.apply(ry_ops)
-    timer = StreamTimer(psi.stream())
-    bras = [N.State(n)]
-
-    def adj_k(k):
-        bras[0].copy_from(psi)
-        sub = ry_ops[:k]
-        return timer.time(lambda: N.adjoint_jacobian(psi, bras, sub, list(range(k))))
-
-    adj_k(4)
-    t1 = min(adj_k(n) for _ in range(2))
-    t2 = min(adj_k(2 * n) for _ in range(2))
-    fused_ms = (t2 - t1) / n
+    # ---------------------------------------------------------------- dominant kernel: fused adjoint pass
+    # One launch reads and writes lambda and the observable state once:
+    # algorithmic bytes = 4 * state bytes per launch, whatever the number of
+    # gates fused into it.
+    adj_avg = adj_ms / max(1, adj_n)
     fused_bytes = 4 * state_bytes
-    fused_gbs = fused_bytes / (fused_ms * 1e-3) / 1e9
-    del psi, bras
+    fused_gbs = fused_bytes / (adj_avg * 1e-3) / 1e9
+    fwd_avg = fwd_ms / max(1, fwd_n)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile) and n == 28:
         with open(tfile) as f:
-            traffic = json.load(f).get("k_adj_dense<double,1>@28q,1obs", {}).get("traffic_bytes")
+            traffic = json.load(f).get(f"k_fused_adj<double>@{n}q", {}).get("traffic_bytes")
+    kernel_stats = {
+        "k_fused_adj": {"launches_per_step": adj_n / args.steps, "avg_ms": round(adj_avg, 4),
+                        "ms_per_step": round(adj_ms / args.steps, 2), "bytes_per_launch": fused_bytes,
+                        "GBps": round(fused_gbs, 1)},
+        "k_fused_pass": {"launches_per_step": fwd_n / args.steps, "avg_ms": round(fwd_avg, 4),
+                         "ms_per_step": round(fwd_ms / args.steps, 2), "bytes_per_launch": 2 * state_bytes,
+                         "GBps": round(2 * state_bytes / (fwd_avg * 1e-3) / 1e9, 1) if fwd_n else None},
+    }
+
+    # ---------------------------------------------------------------- batched circuits
+    batched = None
+    if not args.no_batch:
+        batched = bench_batched(N, args)
+    serving = None
+    if not args.no_serve:
+        serving = bench_serving(N, args)
 
     # ---------------------------------------------------------------- 30q single-gate sweep
     sweep = None
@@ -234,11 +383,14 @@ def main():
         "config": {"workload": f"adjoint Jacobian, {n} qubits c128, {layers} layers RY/RZ + CZ brick, "
                                f"{len(ops)} ops, {n_par} params, <Z0 Z{n - 1}>", "seed": args.seed,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "fused adjoint step (k_adj_dense<double,1>, 1 observable)",
+        "roofline": {"bound": "hbm", "kernel": "k_fused_adj<double> (fused adjoint pass: lambda + 1 observable state)",
                      "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fused_gbs / HBM_PEAK_GBS, 3), "traffic": traffic,
-                     "bytes_per_launch": fused_bytes, "ms_per_launch": round(fused_ms, 4)},
+                     "bytes_per_launch": fused_bytes, "ms_per_launch": round(adj_avg, 4)},
         "cpu_baseline": cpu,
+        "kernels": kernel_stats,
+        "batched": batched,
+        "serving": serving,
         "gate_sweep": sweep,
         "jacobian_checksum": float(np.sum(jac)),
     }

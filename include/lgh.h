@@ -127,6 +127,17 @@ lgh_status lgh_pool_trim(int device);
  * default).  No counterpart in the reference, which applies one kernel per
  * gate; exposed for comparisons and tests. */
 lgh_status lgh_set_fusion(int enabled);
+/* Tile height of the fused passes: register bits (3 or 4) of the forward and
+ * of the adjoint passes; a wavefront holds 2^(6 + bits) amplitudes per state.
+ * Defaults 4 and 3.  No counterpart in the reference; a tuning knob. */
+lgh_status lgh_set_tile_bits(int forward, int adjoint);
+/* Launch timing of the hot kernels with HIP events recorded on their
+ * launch stream (off by default; no counterpart in the reference).  Enabling
+ * or disabling clears what was recorded.  kind 0: fused forward passes,
+ * 1: fused adjoint passes, 2: batched-circuit launches.  lgh_profile_read
+ * waits for the recorded events and returns their count and summed time. */
+lgh_status lgh_profile_enable(int enabled);
+lgh_status lgh_profile_read(int kind, int64_t *launches, double *total_ms);
 /* Streams for callers that overlap independent states (serving). */
 lgh_status lgh_stream_create(int device, void **stream);
 lgh_status lgh_stream_destroy(void *stream);
@@ -212,6 +223,80 @@ lgh_status lgh_sample_bits(const lgh_sv *sv, uint64_t shots, uint64_t seed, uint
 lgh_status lgh_adjoint_jacobian(const lgh_sv *psi, lgh_sv *const *bras, int32_t num_obs,
                                 const lgh_op *ops, int64_t num_ops, const int64_t *trainable,
                                 int64_t num_trainable, int32_t apply_operations, double *jac);
+
+/* ---- batched circuits ---------------------------------------------------
+ * No counterpart in the reference, which runs one circuit per device call
+ * (the closest form is the per-GPU observable batching of batch_obs,
+ * AdjointDiffGPU.hpp:392-473).  A template is the gate structure shared by a
+ * family of circuits that differ only in their parameters, with one Pauli
+ * word as observable; a batch runs many circuits (any mix of templates) in
+ * one persistent launch: forward pass from |0..0>, <P>, and optionally the
+ * adjoint gradient of <P> with respect to every circuit parameter. */
+typedef struct lgh_tmpl lgh_tmpl;
+
+/* ops: the circuit (num_qubits <= 20; every gate single-target dense or
+ * diagonal, e.g. RX/RY/RZ/PhaseShift/CRX/CRY/CRZ/CPhase/IsingZZ/MultiRZ and
+ * fixed 1- and controlled gates).  param_index[i]: index into the circuit's
+ * parameter vector that drives ops[i].params[0] (parameter sharing allowed),
+ * or -1 for a gate whose params are fixed as given.  The observable is the
+ * tensor product of obs_paulis[k] (LGH_GATE_PAULIX/Y/Z) on obs_wires[k]. */
+lgh_status lgh_tmpl_create(lgh_tmpl **out, int32_t num_qubits, const lgh_op *ops, int32_t num_ops,
+                           const int32_t *param_index, const int32_t *obs_paulis, const int32_t *obs_wires,
+                           int32_t obs_len, int32_t device);
+lgh_status lgh_tmpl_destroy(lgh_tmpl *tmpl);
+/* number of parameters (1 + the largest param_index) and of qubits */
+lgh_status lgh_tmpl_info(const lgh_tmpl *tmpl, int32_t *num_params, int32_t *num_qubits);
+
+/* Circuit c (0 <= c < num_circuits) is tmpls[c] with parameters
+ * params[param_offset[c] ...] (host memory, float64).  expval[c] = <P>; if
+ * jac is not NULL, jac[jac_offset[c] + j] = d<P>/d param_j for the template's
+ * parameters.  dtype selects complex64 or complex128 states; stream as for
+ * lgh_sv_create (NULL: the library's stream).  Blocks until the results are
+ * on the host. */
+lgh_status lgh_batch_run(lgh_tmpl *const *tmpls, int32_t num_circuits, lgh_dtype dtype, const double *params,
+                         const int64_t *param_offset, double *expval, double *jac, const int64_t *jac_offset,
+                         void *stream);
+/* Same results through one state per circuit and the single-circuit entry
+ * points (lgh_apply, lgh_expval_pauli_words, lgh_adjoint_jacobian), one
+ * circuit after another: the per-circuit launch form the batch replaces. */
+lgh_status lgh_batch_run_serial(lgh_tmpl *const *tmpls, int32_t num_circuits, lgh_dtype dtype,
+                                const double *params, const int64_t *param_offset, double *expval, double *jac,
+                                const int64_t *jac_offset, void *stream);
+
+/* ---- serving ------------------------------------------------------------
+ * No counterpart in the reference, where each PennyLane execution calls the
+ * device synchronously (lightning_gpu.py:519-555, :638-752); one request is
+ * one such execution.  Worker threads with one stream each take requests
+ * from a queue: circuits of at most batch_max_qubits qubits are batched
+ * together into persistent launches (continuous batching), larger ones run
+ * on states of their own; all work is admitted against a device-memory
+ * budget. */
+typedef struct lgh_server lgh_server;
+typedef struct lgh_request {
+    int32_t num_qubits;
+    const lgh_op *ops;          /* bound parameters; copied at submission */
+    int32_t num_ops;
+    const int32_t *obs_paulis;  /* LGH_GATE_PAULIX/Y/Z */
+    const int32_t *obs_wires;
+    int32_t obs_len;
+    int32_t want_grad;          /* d<P>/d params of every parametrised op, in op order */
+} lgh_request;
+
+/* memory_budget: device bytes the server may hold (0: 90% of free memory);
+ * num_streams: worker threads / streams, 1..16; batch_max_qubits: 0..20. */
+lgh_status lgh_server_create(lgh_server **out, int32_t device, uint64_t memory_budget, int32_t num_streams,
+                             int32_t batch_max_qubits);
+/* Stops the workers after their current work; requests still queued are dropped. */
+lgh_status lgh_server_destroy(lgh_server *server);
+/* Queues a request; rejects one that can never fit the budget. */
+lgh_status lgh_server_submit(lgh_server *server, const lgh_request *request, uint64_t *ticket);
+/* Blocks until the request is done.  grad (nullable) receives grad_len >=
+ * number of parametrised ops entries; timing_ms (nullable, 3 entries):
+ * queueing delay, run time, total latency. */
+lgh_status lgh_server_wait(lgh_server *server, uint64_t ticket, double *expval, double *grad, int64_t grad_len,
+                           double *timing_ms);
+lgh_status lgh_server_stats(lgh_server *server, uint64_t *batches, uint64_t *batched_requests,
+                            uint64_t *single_requests, uint64_t *budget);
 
 #ifdef __cplusplus
 }

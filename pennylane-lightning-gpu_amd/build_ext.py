@@ -35,6 +35,12 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function", "-I", INCLUDE, "-I", os.path.join(CSRC, "lgh"),
 ]
+# The fused kernels branch on per-gate descriptor fields that are uniform
+# across the wavefront; leaving those regions unstructurized keeps the
+# register arrays in place across the branches instead of duplicating them.
+FILE_FLAGS = {
+    "lgh_fused.hip": ["-mllvm", "-structurizecfg-skip-uniform-regions=true"],
+}
 
 
 def _run(cmd):
@@ -61,7 +67,7 @@ def build_lgh(force=True, jobs=8):
         o = os.path.join(BUILD, os.path.basename(s) + ".o")
         objs.append(o)
         if force or not _newer(o, [s] + headers):
-            jobs_list.append([HIPCC, *HIP_FLAGS, "-c", s, "-o", o])
+            jobs_list.append([HIPCC, *HIP_FLAGS, *FILE_FLAGS.get(os.path.basename(s), []), "-c", s, "-o", o])
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, jobs_list))
     out = os.path.join(PKG, "liblgh.so")

@@ -26,7 +26,9 @@
 #include <cstring>
 #include <map>
 
-#include "lgh_fused.hpp"
+#include "lgh_fused_dev.hpp"
+
+using namespace lgh::fz;
 
 namespace lgh {
 
@@ -34,12 +36,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
-constexpr int RB = 4;              // register bits per pass
-constexpr int NREG = 1 << RB;      // amplitudes per lane
-constexpr int kLaneBits = 6;
-constexpr int kMaxSlots = 32;      // trainable gates per pass
 
 std::atomic<int> g_fusion{1};
+std::atomic<int> g_rb_fwd{kRBForward}, g_rb_adj{kRBAdjoint};
 
 struct Use {
     uint64_t diag = 0;    // bits used diagonally (controls, phases)
@@ -60,16 +59,22 @@ Use uses_of(const KOp &k) {
 } // namespace
 
 void set_fusion(int enabled) { g_fusion.store(enabled ? 1 : 0); }
+void set_tile_bits(int forward, int adjoint) {
+    LGH_REQUIRE((forward == 3 || forward == 4) && (adjoint == 3 || adjoint == 4), "tile bits are 3 or 4");
+    g_rb_fwd.store(forward);
+    g_rb_adj.store(adjoint);
+}
 int fusion_enabled() { return g_fusion.load(); }
 
-bool fusable(const KOp &k, int n) {
-    if (n < kLaneBits + RB) return false;
+bool fusable(const KOp &k, int n, int rb) {
+    if (n < kLaneBits + rb) return false;
     if (k.kind == KOp::DIAG) return true;
     return k.kind == KOp::DENSE && k.k == 1;
 }
 
 // Greedy list scheduling of `ops` (already compiled) into passes.
-std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot) {
+std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot,
+                                  int rb) {
     const int N = static_cast<int>(ops.size());
     std::vector<Use> use(N);
     for (int i = 0; i < N; ++i) use[i] = uses_of(ops[i]);
@@ -123,7 +128,7 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
     while (remaining > 0) {
         std::sort(ready.begin(), ready.end());
         // a ready gate that cannot be fused runs on its own
-        if (!fusable(ops[ready.front()], n)) {
+        if (!fusable(ops[ready.front()], n, rb)) {
             const int i = ready.front();
             ready.erase(ready.begin());
             Pass p;
@@ -138,15 +143,15 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
         uint64_t reg = 0;
         int nreg = 0;
         for (int i : ready) {
-            if (!fusable(ops[i], n) || ops[i].kind != KOp::DENSE) continue;
+            if (!fusable(ops[i], n, rb) || ops[i].kind != KOp::DENSE) continue;
             const uint64_t t = use[i].nondiag & ~lane_mask;
-            if (t && !(reg & t) && nreg < RB) {
+            if (t && !(reg & t) && nreg < rb) {
                 reg |= t;
                 ++nreg;
             }
         }
-        // pad with the lowest unused bits so that every pass has RB register bits
-        for (int b = kLaneBits; b < n && nreg < RB; ++b)
+        // pad with the lowest unused bits so that every pass has rb register bits
+        for (int b = kLaneBits; b < n && nreg < rb; ++b)
             if (!(reg & (1ull << b))) {
                 reg |= 1ull << b;
                 ++nreg;
@@ -158,7 +163,7 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
             std::sort(ready.begin(), ready.end());
             for (size_t r = 0; r < ready.size(); ++r) {
                 const int i = ready[r];
-                if (!fusable(ops[i], n)) continue;
+                if (!fusable(ops[i], n, rb)) continue;
                 const uint64_t t = use[i].nondiag & ~lane_mask;
                 if (t & ~reg) continue;
                 const bool tr = !trainable_slot.empty() && trainable_slot[i] >= 0;
@@ -184,222 +189,112 @@ using namespace lgh;
 
 namespace {
 
-// Device description of one fused gate.
-struct FOp {
-    int32_t kind;  // 0 = dense on lane bit, 1 = dense on register bit, 2 = diagonal
-    int32_t loc;   // lane bit or register bit
-    int32_t slot;  // trainable slot in the pass, -1 if none
-    int32_t gkind; // generator: 0 dense 2x2, 1 diagonal
-    uint64_t cmask, pmask;
-    cx<double> m[4]; // U (dense) or (d_even, d_odd) in m[0], m[1]
-    cx<double> g[4]; // generator (dense) or (g_even, g_odd) in g[0].re, g[1].re
-};
-
-struct PassDesc {
-    int32_t nops;
-    int32_t nslots;
-    int32_t reg_bits[RB];
-    uint64_t reg_off[NREG]; // index offset of each register
-    Fixed tile_fx;          // zero-insertion positions of lane and register bits
-};
-
-template <class R> __device__ __forceinline__ cx<R> lo(cx<double> z) { return cx<R>{R(z.re), R(z.im)}; }
-
-// Apply a dense 2x2 on register bit B to all NREG registers.
-template <class R, int B>
-__device__ __forceinline__ void reg_gate(cx<R> (&v)[NREG], const uint64_t (&idx)[NREG], uint64_t cmask, cx<R> m00,
-                                         cx<R> m01, cx<R> m10, cx<R> m11) {
-#pragma unroll
-    for (int k = 0; k < NREG; ++k) {
-        if (k & (1 << B)) continue;
-        const int k1 = k | (1 << B);
-        if ((idx[k] & cmask) != cmask) continue;
-        const cx<R> a = v[k], b = v[k1];
-        v[k] = cfma(m01, b, cmul(m00, a));
-        v[k1] = cfma(m11, b, cmul(m10, a));
-    }
-}
-
-template <class R>
-__device__ __forceinline__ void reg_gate_dispatch(int b, cx<R> (&v)[NREG], const uint64_t (&idx)[NREG], uint64_t cm,
-                                                  cx<R> m00, cx<R> m01, cx<R> m10, cx<R> m11) {
-    switch (b) {
-    case 0: reg_gate<R, 0>(v, idx, cm, m00, m01, m10, m11); break;
-    case 1: reg_gate<R, 1>(v, idx, cm, m00, m01, m10, m11); break;
-    case 2: reg_gate<R, 2>(v, idx, cm, m00, m01, m10, m11); break;
-    default: reg_gate<R, 3>(v, idx, cm, m00, m01, m10, m11); break;
-    }
-}
-
-// Dense 2x2 on lane bit l: partner in lane ^ 2^l.
-template <class R>
-__device__ __forceinline__ void lane_gate(int l, int lane, cx<R> (&v)[NREG], const uint64_t (&idx)[NREG],
-                                          uint64_t cmask, cx<R> m00, cx<R> m01, cx<R> m10, cx<R> m11) {
-    const int mask = 1 << l;
-    const bool hi = (lane & mask) != 0;
-    const cx<R> cs = hi ? m11 : m00, cp = hi ? m10 : m01;
-#pragma unroll
-    for (int k = 0; k < NREG; ++k) {
-        const cx<R> p = shfl_xor_cx(v[k], mask);
-        if ((idx[k] & cmask) == cmask) v[k] = cfma(cp, p, cmul(cs, v[k]));
-    }
-}
-
-template <class R>
-__device__ __forceinline__ void diag_gate(cx<R> (&v)[NREG], const uint64_t (&idx)[NREG], uint64_t cmask,
-                                          uint64_t pmask, cx<R> de, cx<R> dodd) {
-#pragma unroll
-    for (int k = 0; k < NREG; ++k) {
-        if ((idx[k] & cmask) != cmask) continue;
-        v[k] = cmul(v[k], (__popcll(idx[k] & pmask) & 1) ? dodd : de);
-    }
-}
-
-template <class R>
-__device__ __forceinline__ void apply_fop(const FOp &op, int lane, cx<R> (&v)[NREG], const uint64_t (&idx)[NREG],
-                                          const cx<double> (&m)[4]) {
-    if (op.kind == 2) {
-        diag_gate<R>(v, idx, op.cmask, op.pmask, lo<R>(m[0]), lo<R>(m[1]));
-    } else if (op.kind == 1) {
-        reg_gate_dispatch<R>(op.loc, v, idx, op.cmask, lo<R>(m[0]), lo<R>(m[1]), lo<R>(m[2]), lo<R>(m[3]));
-    } else {
-        lane_gate<R>(op.loc, lane, v, idx, op.cmask, lo<R>(m[0]), lo<R>(m[1]), lo<R>(m[2]), lo<R>(m[3]));
-    }
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-// Im sum_i conj(b_i) (G a)_i for a dense generator on register bit B.
-template <class R, int B>
-__device__ __forceinline__ double gen_dot_reg(const FOp &op, const cx<R> (&a)[NREG], const cx<R> (&b)[NREG],
-                                              const uint64_t (&idx)[NREG]) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < NREG; ++k) {
-        if ((idx[k] & op.cmask) != op.cmask) continue;
-        const bool hi = (k >> B) & 1;
-        const cx<R> p = a[k ^ (1 << B)];
-        const cx<double> gs = hi ? op.g[3] : op.g[0], gp = hi ? op.g[2] : op.g[1];
-        const cx<double> ga = cfma(gp, cx<double>{double(p.re), double(p.im)},
-                                   cmul(gs, cx<double>{double(a[k].re), double(a[k].im)}));
-        s += double(b[k].re) * ga.im - double(b[k].im) * ga.re;
-    }
-    return s;
-}
-
-// Im sum_i conj(b_i) (G a)_i over the tile for one trainable gate.
-template <class R>
-__device__ __forceinline__ double gen_dot(const FOp &op, int lane, const cx<R> (&a)[NREG], const cx<R> (&b)[NREG],
-                                          const uint64_t (&idx)[NREG]) {
-    double s = 0.0;
-    if (op.gkind == 1) { // diagonal generator
-#pragma unroll
-        for (int k = 0; k < NREG; ++k) {
-            if ((idx[k] & op.cmask) != op.cmask) continue;
-            const double gv = (__popcll(idx[k] & op.pmask) & 1) ? op.g[1].re : op.g[0].re;
-            s += gv * (double(b[k].re) * a[k].im - double(b[k].im) * a[k].re);
-        }
-        return s;
-    }
-    if (op.kind == 1) { // register bit
-        switch (op.loc) {
-        case 0: return gen_dot_reg<R, 0>(op, a, b, idx);
-        case 1: return gen_dot_reg<R, 1>(op, a, b, idx);
-        case 2: return gen_dot_reg<R, 2>(op, a, b, idx);
-        default: return gen_dot_reg<R, 3>(op, a, b, idx);
-        }
-    }
-    // lane bit
-    const int mask = 1 << op.loc;
-    const bool hi = (lane & mask) != 0;
-    const cx<double> gs = hi ? op.g[3] : op.g[0], gp = hi ? op.g[2] : op.g[1];
-#pragma unroll
-    for (int k = 0; k < NREG; ++k) {
-        const cx<R> p = shfl_xor_cx(a[k], mask);
-        if ((idx[k] & op.cmask) != op.cmask) continue;
-        const cx<double> ga =
-            cfma(gp, cx<double>{double(p.re), double(p.im)}, cmul(gs, cx<double>{double(a[k].re), double(a[k].im)}));
-        s += double(b[k].re) * ga.im - double(b[k].im) * ga.re;
-    }
-    return s;
-}
-
 // Forward pass: every wavefront processes tiles [w*chunk, (w+1)*chunk).
-template <class R>
+template <class R, int RB>
 __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, uint64_t ntiles, uint64_t chunk,
-                                                       const PassDesc *__restrict__ pd_g,
-                                                       const FOp *__restrict__ ops_g) {
+                                                       const PassDesc *__restrict__ pd,
+                                                       const FOp *__restrict__ ops) {
     const int lane = threadIdx.x & 63;
+    const CFOp *cops = (const CFOp *)ops;
     const uint64_t wave = xcd_block() * kWavesPerBlock + (threadIdx.x >> 6);
-    const PassDesc pd = *pd_g;
     const uint64_t t0 = wave * chunk, t1 = min(t0 + chunk, ntiles);
+    const int nops = pd->nops;
+    constexpr int NR = 1 << RB;
+    uint64_t roff[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) roff[k] = pd->reg_off[k];
+    int rs[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) rs[r] = pd->reg_sorted[r];
     for (uint64_t t = t0; t < t1; ++t) {
-        const uint64_t base = expand(t, pd.tile_fx) | uint64_t(lane);
-        uint64_t idx[NREG];
-        cx<R> v[NREG];
+        const uint64_t base = tile_base<RB>(t, rs) | uint64_t(lane);
+        cx<R> v[NR];
+        cx<R> *p = sv + base; // register bits are above the lane bits: | equals +
 #pragma unroll
-        for (int k = 0; k < NREG; ++k) {
-            idx[k] = base | pd.reg_off[k];
-            v[k] = ldnt(sv + idx[k]);
+        for (int k = 0; k < NR; ++k) v[k] = ldnt(p + roff[k]);
+        const CFOp *opp = cops;
+        FOp cur = load_op(opp);
+        for (int o = 0; o < nops; ++o) {
+            const FOp nxt = load_op(opp + (o + 1 < nops ? o + 1 : o)); // prefetch the next descriptor
+            apply_fop<R, NR>(cur, lane, base, v);
+            cur = nxt;
         }
-        for (int o = 0; o < pd.nops; ++o) {
-            const FOp &op = ops_g[o];
-            const cx<double> m[4] = {op.m[0], op.m[1], op.m[2], op.m[3]};
-            apply_fop<R>(op, lane, v, idx, m);
-        }
+        // recompute the store addresses instead of keeping NR of them live
+        asm volatile("" : "+v"(p));
 #pragma unroll
-        for (int k = 0; k < NREG; ++k) stnt(sv + idx[k], v[k]);
+        for (int k = 0; k < NR; ++k) stnt(p + roff[k], v[k]);
     }
 }
 
 // Reverse pass of the adjoint sweep on lambda and one observable state.  The
-// ops of the pass are stored in reverse order with U^dagger matrices.
-template <class R>
+// ops of the pass are stored in reverse order with U^dagger matrices.  Each
+// lane accumulates its overlap contributions in its own LDS column; the
+// reduction over lanes and wavefronts happens once at the end.
+template <class R, int RB>
 __global__ __launch_bounds__(kBlock) void k_fused_adj(cx<R> *__restrict__ lam, cx<R> *__restrict__ bra,
-                                                      uint64_t ntiles, uint64_t chunk,
-                                                      const PassDesc *__restrict__ pd_g,
-                                                      const FOp *__restrict__ ops_g, double *__restrict__ partials) {
-    __shared__ double acc[kWavesPerBlock][kMaxSlots];
+                                                      uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
+                                                      const FOp *__restrict__ ops, const FGen *__restrict__ gens,
+                                                      double *__restrict__ partials) {
+    __shared__ double acc[kMaxSlots][kBlock];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int s = lane; s < kMaxSlots; s += 64) acc[w][s] = 0.0;
+    const CFOp *cops = (const CFOp *)ops;
+    const CFGen *cgens = (const CFGen *)gens;
+    const int nslots = pd->nslots;
+    for (int s = 0; s < nslots; ++s) acc[s][threadIdx.x] = 0.0;
     const uint64_t wave = xcd_block() * kWavesPerBlock + w;
-    const PassDesc pd = *pd_g;
     const uint64_t t0 = wave * chunk, t1 = min(t0 + chunk, ntiles);
+    const int nops = pd->nops;
+    constexpr int NR = 1 << RB;
+    uint64_t roff[NR];
+#pragma unroll
+    for (int k = 0; k < NR; ++k) roff[k] = pd->reg_off[k];
+    int rs[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) rs[r] = pd->reg_sorted[r];
     for (uint64_t t = t0; t < t1; ++t) {
-        const uint64_t base = expand(t, pd.tile_fx) | uint64_t(lane);
-        uint64_t idx[NREG];
-        cx<R> a[NREG], b[NREG];
+        const uint64_t base = tile_base<RB>(t, rs) | uint64_t(lane);
+        cx<R> a[NR], b[NR];
+        cx<R> *pa = lam + base, *pb = bra + base; // register bits are above the lane bits
 #pragma unroll
-        for (int k = 0; k < NREG; ++k) {
-            idx[k] = base | pd.reg_off[k];
-            a[k] = ldnt(lam + idx[k]);
-            b[k] = ldnt(bra + idx[k]);
+        for (int k = 0; k < NR; ++k) {
+            a[k] = ldnt(pa + roff[k]);
+            b[k] = ldnt(pb + roff[k]);
         }
-        for (int o = 0; o < pd.nops; ++o) {
-            const FOp &op = ops_g[o];
-            if (op.slot >= 0) {
-                const double d = wave_sum(gen_dot<R>(op, lane, a, b, idx));
-                if (lane == 0) acc[w][op.slot] += d;
+        const CFOp *opp = cops;
+        FOp cur = load_op(opp);
+        for (int o = 0; o < nops; ++o) {
+            const FOp nxt = load_op(opp + (o + 1 < nops ? o + 1 : o));
+            const int slot = f_slot(cur);
+            if (f_kind(cur) != kDiag && f_plain(cur)) {
+                if (slot >= 0) {
+                    const FGen g = load_gen(cgens + slot);
+                    acc[slot][threadIdx.x] += adj_plain_dense<R, NR, true>(cur, &g, lane, a, b);
+                } else {
+                    adj_plain_dense<R, NR, false>(cur, nullptr, lane, a, b);
+                }
+            } else {
+                if (slot >= 0) {
+                    const FGen g = load_gen(cgens + slot);
+                    acc[slot][threadIdx.x] += gen_dot<R, NR>(cur, g, lane, base, a, b);
+                }
+                apply_fop<R, NR>(cur, lane, base, a);
+                apply_fop<R, NR>(cur, lane, base, b);
             }
-            const cx<double> m[4] = {op.m[0], op.m[1], op.m[2], op.m[3]};
-            apply_fop<R>(op, lane, a, idx, m);
-            apply_fop<R>(op, lane, b, idx, m);
+            cur = nxt;
         }
+        asm volatile("" : "+v"(pa), "+v"(pb));
 #pragma unroll
-        for (int k = 0; k < NREG; ++k) {
-            stnt(lam + idx[k], a[k]);
-            stnt(bra + idx[k], b[k]);
+        for (int k = 0; k < NR; ++k) {
+            stnt(pa + roff[k], a[k]);
+            stnt(pb + roff[k], b[k]);
         }
     }
     __syncthreads();
-    for (int s = threadIdx.x; s < pd.nslots; s += kBlock) {
-        double v = 0.0;
-        for (int q = 0; q < kWavesPerBlock; ++q) v += acc[q][s];
-        partials[size_t(s) * gridDim.x + blockIdx.x] = v;
+    // slot s: wavefront w sums the 256 columns in a fixed order
+    for (int s = w; s < nslots; s += kWavesPerBlock) {
+        double v = acc[s][lane] + acc[s][lane + 64] + acc[s][lane + 128] + acc[s][lane + 192];
+        v = wave_sum(v);
+        if (lane == 0) partials[size_t(s) * gridDim.x + blockIdx.x] = v;
     }
 }
 
@@ -422,82 +317,175 @@ __global__ __launch_bounds__(kBlock) void k_slot_store(const double *__restrict_
     }
 }
 
-// Host: device description of a pass.
-struct Built {
-    PassDesc pd;
-    std::vector<FOp> ops;
+// host form of FOp before its fields are packed into meta
+struct HOp {
+    uint32_t kind = 0, gkind = 0, dneg = 0;
+    int loc = 0, slot = -1;
+    uint64_t cm_reg = 0, pm_reg = 0; // control / parity bits among the register bits
+    FOp f{};
 };
 
-Built build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse,
-                 const std::vector<KGen> *gens, const std::vector<int> *slot_of) {
-    Built b{};
-    std::vector<int> tile_bits;
-    for (int l = 0; l < kLaneBits; ++l) tile_bits.push_back(l);
-    for (int r = 0; r < RB; ++r) {
-        b.pd.reg_bits[r] = p.reg_bits[r];
-        tile_bits.push_back(p.reg_bits[r]);
+FOp pack(const HOp &h, const uint64_t (&reg_off)[kMaxNR], int nr) {
+    FOp f = h.f;
+    bool real = h.kind != kDiag;
+    for (int q = 0; q < 4 && real; ++q) real = f.m[q].im == 0.0;
+    f.meta = h.kind | uint32_t(h.loc) << 4 | uint32_t(h.slot + 1) << 8 | h.gkind << 16 | h.dneg << 20 |
+             uint32_t(real) << 21;
+    uint32_t rok = 0, rpar = 0;
+    for (int k = 0; k < kMaxNR; ++k) {
+        // registers beyond nr do not exist; their bits read as "controls hold"
+        if (k >= nr || (reg_off[k] & h.cm_reg) == h.cm_reg) rok |= 1u << k;
+        if (k >= nr) continue;
+        if (__builtin_popcountll(reg_off[k] & h.pm_reg) & 1) rpar |= 1u << k;
     }
-    b.pd.tile_fx = make_fixed(tile_bits);
-    for (int k = 0; k < NREG; ++k) {
+    f.rbits = rok | rpar << 16;
+    return f;
+}
+
+bool is_neg_diag(const KOp &k) {
+    return k.kind == KOp::DIAG && std::abs(k.d_even + 1.0) < 1e-300 && std::abs(k.d_odd + 1.0) < 1e-300;
+}
+
+} // namespace
+
+Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse,
+                          const std::vector<KGen> *gens, const std::vector<int> *slot_of, int rb, bool merge_forward) {
+    (void)n;
+    LGH_REQUIRE(rb >= 1 && rb <= kMaxRB, "fused pass: bad register-bit count");
+    const int nr = 1 << rb;
+    Built b{};
+    b.pd.rb = rb;
+    uint64_t regmask = 0;
+    for (int r = 0; r < rb; ++r) {
+        LGH_REQUIRE(p.reg_bits[r] >= kLaneBits, "fused pass: register bit below the lane bits");
+        b.pd.reg_bits[r] = p.reg_bits[r];
+        b.pd.reg_sorted[r] = p.reg_bits[r];
+        regmask |= 1ull << p.reg_bits[r];
+    }
+    std::sort(b.pd.reg_sorted, b.pd.reg_sorted + rb);
+    const uint64_t tilemask = regmask | ((1ull << kLaneBits) - 1);
+    for (int k = 0; k < nr; ++k) {
         uint64_t o = 0;
-        for (int r = 0; r < RB; ++r)
+        for (int r = 0; r < rb; ++r)
             if ((k >> r) & 1) o |= 1ull << p.reg_bits[r];
         b.pd.reg_off[k] = o;
     }
+    auto reg_loc = [&](int bit) {
+        for (int r = 0; r < rb; ++r)
+            if (p.reg_bits[r] == bit) return r;
+        return -1;
+    };
     std::vector<int> order = p.ops;
     if (reverse) std::reverse(order.begin(), order.end());
+    // forward passes merge consecutive single-qubit gates on the same tile bit
+    // into one 2x2 (untrained passes only)
+    const bool merge = merge_forward && slot_of == nullptr;
+    std::vector<HOp> hops;
+    std::vector<int> last_single(64, -1); // per bit: index in hops of a mergeable gate
     int nslots = 0;
     for (int i : order) {
         const KOp &k = kops[size_t(i)];
-        FOp f{};
-        f.slot = -1;
+        HOp h;
+        FOp &f = h.f;
+        int single_bit = -1; // uncontrolled gate acting on exactly one tile bit
         if (k.kind == KOp::DIAG) {
-            f.kind = 2;
-            f.cmask = k.cmask;
-            f.pmask = k.pmask;
+            h.kind = kDiag;
+            h.cm_reg = k.cmask & regmask;
+            f.cm_rest = k.cmask & ~regmask;
+            h.pm_reg = k.pmask & regmask;
+            f.pm_rest = k.pmask & ~regmask;
             f.m[0] = to_cx<double>(k.d_even);
             f.m[1] = to_cx<double>(k.d_odd);
+            h.dneg = is_neg_diag(k) ? 1 : 0;
+            if (k.cmask == 0 && __builtin_popcountll(k.pmask) == 1 && (k.pmask & tilemask))
+                single_bit = __builtin_ctzll(k.pmask);
         } else {
             const int t = k.tbits[0];
-            f.cmask = k.cmask;
+            h.cm_reg = k.cmask & regmask;
+            f.cm_rest = k.cmask & ~regmask;
             if (t < kLaneBits) {
-                f.kind = 0;
-                f.loc = t;
+                h.kind = kLaneDense;
+                h.loc = t;
             } else {
-                f.kind = 1;
-                f.loc = -1;
-                for (int r = 0; r < RB; ++r)
-                    if (p.reg_bits[r] == t) f.loc = r;
-                LGH_REQUIRE(f.loc >= 0, "fused pass: target is not a register bit");
+                h.kind = kRegDense;
+                h.loc = reg_loc(t);
+                LGH_REQUIRE(h.loc >= 0, "fused pass: target is not a register bit");
             }
             for (int q = 0; q < 4; ++q) f.m[q] = to_cx<double>(k.mat[size_t(q)]);
+            if (k.cmask == 0) single_bit = t;
         }
-        if (slot_of && (*slot_of)[size_t(i)] >= 0) {
+        const bool trained = slot_of && (*slot_of)[size_t(i)] >= 0;
+        if (trained) {
             const KGen &g = (*gens)[size_t(i)];
-            f.slot = nslots++;
+            FGen fg{};
+            h.slot = nslots++;
             if (g.mat.empty()) {
-                f.gkind = 1;
-                f.g[0] = cx<double>{g.g_even, 0.0};
-                f.g[1] = cx<double>{g.g_odd, 0.0};
+                h.gkind = kGenDiag;
+                fg.g[0] = cx<double>{g.g_even, 0.0};
+                fg.g[1] = cx<double>{g.g_odd, 0.0};
             } else {
-                f.gkind = 0;
-                for (int q = 0; q < 4; ++q) f.g[q] = to_cx<double>(g.mat[size_t(q)]);
+                h.gkind = kGenDense;
+                for (int q = 0; q < 4; ++q) fg.g[q] = to_cx<double>(g.mat[size_t(q)]);
+                const auto &G = g.mat;
+                const std::complex<double> i1(0, 1);
+                if (G[0] == 0.0 && G[3] == 0.0 && G[1] == 1.0 && G[2] == 1.0) h.gkind = kGenX;
+                else if (G[0] == 0.0 && G[3] == 0.0 && G[1] == -i1 && G[2] == i1) h.gkind = kGenY;
             }
+            b.gens.push_back(fg);
         }
-        b.ops.push_back(f);
+        // bits this gate touches invalidate pending merges on them
+        const uint64_t touched = k.kind == KOp::DIAG ? (k.cmask | k.pmask) : (k.cmask | (1ull << k.tbits[0]));
+        if (merge && single_bit >= 0 && last_single[size_t(single_bit)] >= 0) {
+            // fold into the earlier gate on the same bit: M <- M_new * M_old
+            HOp &prev = hops[size_t(last_single[size_t(single_bit)])];
+            const cx<double> zero{0, 0};
+            cx<double> mn[4], mo[4];
+            if (h.kind == kDiag) {
+                mn[0] = f.m[0]; mn[1] = zero; mn[2] = zero; mn[3] = f.m[1];
+            } else {
+                for (int q = 0; q < 4; ++q) mn[q] = f.m[q];
+            }
+            if (prev.kind == kDiag) {
+                mo[0] = prev.f.m[0]; mo[1] = zero; mo[2] = zero; mo[3] = prev.f.m[1];
+            } else {
+                for (int q = 0; q < 4; ++q) mo[q] = prev.f.m[q];
+            }
+            auto mul = [](cx<double> x, cx<double> y) { return cx<double>{x.re * y.re - x.im * y.im, x.re * y.im + x.im * y.re}; };
+            auto add = [](cx<double> x, cx<double> y) { return cx<double>{x.re + y.re, x.im + y.im}; };
+            const cx<double> r[4] = {add(mul(mn[0], mo[0]), mul(mn[1], mo[2])), add(mul(mn[0], mo[1]), mul(mn[1], mo[3])),
+                                     add(mul(mn[2], mo[0]), mul(mn[3], mo[2])), add(mul(mn[2], mo[1]), mul(mn[3], mo[3]))};
+            const int bit = single_bit;
+            prev = HOp{};
+            if (bit < kLaneBits) {
+                prev.kind = kLaneDense;
+                prev.loc = bit;
+            } else {
+                prev.kind = kRegDense;
+                prev.loc = reg_loc(bit);
+            }
+            for (int q = 0; q < 4; ++q) prev.f.m[q] = r[q];
+            continue;
+        }
+        for (int bit = 0; bit < 64; ++bit)
+            if (touched & (1ull << bit)) last_single[size_t(bit)] = -1;
+        hops.push_back(h);
+        if (merge && single_bit >= 0) last_single[size_t(single_bit)] = int(hops.size()) - 1;
     }
+    for (const HOp &h : hops) b.ops.push_back(pack(h, b.pd.reg_off, nr));
     b.pd.nops = static_cast<int32_t>(b.ops.size());
     b.pd.nslots = nslots;
     return b;
 }
 
+namespace {
+
 struct Geometry {
     uint64_t ntiles, chunk;
     unsigned blocks;
 };
-Geometry geometry(int n, int device) {
+Geometry geometry(int n, int device, int rb) {
     Geometry g{};
-    g.ntiles = 1ull << (n - kLaneBits - RB);
+    g.ntiles = 1ull << (n - kLaneBits - rb);
     // enough wavefronts to fill the chip several times over
     const uint64_t max_waves = uint64_t(std::max(1, sm_count(device))) * 32;
     uint64_t waves = std::min<uint64_t>(g.ntiles, max_waves);
@@ -508,23 +496,26 @@ Geometry geometry(int n, int device) {
     return g;
 }
 
-template <class R> void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pass> &passes) {
+template <class R>
+void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pass> &passes, int rb) {
     StreamCtx *ctx = sv->ctx;
-    const Geometry geo = geometry(sv->n, sv->device);
+    const Geometry geo = geometry(sv->n, sv->device, rb);
     for (const Pass &p : passes) {
         if (p.standalone) {
             launch_kop(sv, kops[size_t(p.ops[0])]);
             continue;
         }
-        Built b = build_pass(p, kops, sv->n, false, nullptr, nullptr);
+        Built b = build_pass(p, kops, sv->n, false, nullptr, nullptr, rb);
         const size_t pb = (sizeof(PassDesc) + 255) & ~size_t(255);
         std::vector<unsigned char> blob(pb + b.ops.size() * sizeof(FOp));
         std::memcpy(blob.data(), &b.pd, sizeof(PassDesc));
         std::memcpy(blob.data() + pb, b.ops.data(), b.ops.size() * sizeof(FOp));
         unsigned char *d = static_cast<unsigned char *>(ctx->upload(blob.data(), blob.size()));
-        hipLaunchKernelGGL(k_fused_pass<R>, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream,
-                           static_cast<cx<R> *>(sv->data), geo.ntiles, geo.chunk,
-                           reinterpret_cast<const PassDesc *>(d), reinterpret_cast<const FOp *>(d + pb));
+        ProfScope prof(kProfFusedPass, ctx->stream);
+        auto kern = rb == 4 ? k_fused_pass<R, 4> : k_fused_pass<R, 3>;
+        hipLaunchKernelGGL(kern, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream, static_cast<cx<R> *>(sv->data),
+                           geo.ntiles, geo.chunk, reinterpret_cast<const PassDesc *>(d),
+                           reinterpret_cast<const FOp *>(d + pb));
         LGH_LAUNCH_CHECK();
     }
 }
@@ -534,15 +525,16 @@ template <class R> void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, co
 namespace lgh {
 
 void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops) {
-    const std::vector<Pass> passes = schedule_passes(kops, sv->n, {});
+    const int rb = g_rb_fwd.load();
+    const std::vector<Pass> passes = schedule_passes(kops, sv->n, {}, rb);
     if (sv->dtype == LGH_C128)
-        run_forward<double>(sv, kops, passes);
+        run_forward<double>(sv, kops, passes, rb);
     else
-        run_forward<float>(sv, kops, passes);
+        run_forward<float>(sv, kops, passes, rb);
 }
 
 int count_passes(const std::vector<KOp> &kops, int n) {
-    return static_cast<int>(schedule_passes(kops, n, {}).size());
+    return static_cast<int>(schedule_passes(kops, n, {}, g_rb_fwd.load()).size());
 }
 
 template <class R>
@@ -553,8 +545,9 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
     // are scheduled on that order and executed backwards.
     StreamCtx *ctx = lam->ctx;
     const int n = lam->n;
-    const std::vector<Pass> passes = schedule_passes(udag, n, slot_col);
-    const Geometry geo = geometry(n, lam->device);
+    const int rb = g_rb_adj.load();
+    const std::vector<Pass> passes = schedule_passes(udag, n, slot_col, rb);
+    const Geometry geo = geometry(n, lam->device, rb);
     for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
         const Pass &p = *it;
         if (p.standalone) {
@@ -563,7 +556,7 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
                        factor[size_t(i)]);
             continue;
         }
-        Built b = build_pass(p, udag, n, true, &gens, &slot_col);
+        Built b = build_pass(p, udag, n, true, &gens, &slot_col, rb);
         std::vector<int64_t> cols;
         std::vector<double> facs;
         for (auto oi = p.ops.rbegin(); oi != p.ops.rend(); ++oi)
@@ -573,21 +566,28 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
             }
         const size_t pb = (sizeof(PassDesc) + 255) & ~size_t(255);
         const size_t ob = (b.ops.size() * sizeof(FOp) + 255) & ~size_t(255);
+        const size_t gb = (b.gens.size() * sizeof(FGen) + 255) & ~size_t(255);
         const size_t cb = (cols.size() * 8 + 255) & ~size_t(255);
-        std::vector<unsigned char> blob(pb + ob + 2 * cb + 256);
+        std::vector<unsigned char> blob(pb + ob + gb + 2 * cb + 256);
         std::memcpy(blob.data(), &b.pd, sizeof(PassDesc));
         std::memcpy(blob.data() + pb, b.ops.data(), b.ops.size() * sizeof(FOp));
-        std::memcpy(blob.data() + pb + ob, cols.data(), cols.size() * 8);
-        std::memcpy(blob.data() + pb + ob + cb, facs.data(), facs.size() * 8);
+        std::memcpy(blob.data() + pb + ob, b.gens.data(), b.gens.size() * sizeof(FGen));
+        std::memcpy(blob.data() + pb + ob + gb, cols.data(), cols.size() * 8);
+        std::memcpy(blob.data() + pb + ob + gb + cb, facs.data(), facs.size() * 8);
         unsigned char *d = static_cast<unsigned char *>(ctx->upload(blob.data(), blob.size()));
-        hipLaunchKernelGGL(k_fused_adj<R>, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream,
-                           static_cast<cx<R> *>(lam->data), static_cast<cx<R> *>(bra->data), geo.ntiles, geo.chunk,
-                           reinterpret_cast<const PassDesc *>(d), reinterpret_cast<const FOp *>(d + pb), partials);
-        LGH_LAUNCH_CHECK();
+        {
+            ProfScope prof(kProfFusedAdj, ctx->stream);
+            auto kern = rb == 4 ? k_fused_adj<R, 4> : k_fused_adj<R, 3>;
+            hipLaunchKernelGGL(kern, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream,
+                               static_cast<cx<R> *>(lam->data), static_cast<cx<R> *>(bra->data), geo.ntiles,
+                               geo.chunk, reinterpret_cast<const PassDesc *>(d), reinterpret_cast<const FOp *>(d + pb),
+                               reinterpret_cast<const FGen *>(d + pb + ob), partials);
+            LGH_LAUNCH_CHECK();
+        }
         if (!cols.empty()) {
             hipLaunchKernelGGL(k_slot_store, dim3(unsigned(cols.size())), dim3(kBlock), 0, ctx->stream, partials,
-                               int(geo.blocks), int(cols.size()), reinterpret_cast<const int64_t *>(d + pb + ob),
-                               reinterpret_cast<const double *>(d + pb + ob + cb), jac_row_dev);
+                               int(geo.blocks), int(cols.size()), reinterpret_cast<const int64_t *>(d + pb + ob + gb),
+                               reinterpret_cast<const double *>(d + pb + ob + gb + cb), jac_row_dev);
             LGH_LAUNCH_CHECK();
         }
     }
@@ -603,12 +603,17 @@ void fused_adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, c
 }
 
 size_t fused_partials_bytes(int n, int device) {
-    if (n < kLaneBits + RB) return 0;
-    return sizeof(double) * size_t(kMaxSlots) * geometry(n, device).blocks;
+    // sized for the smaller tile, which has more workgroups
+    if (n < kLaneBits + 3) return 0;
+    return sizeof(double) * size_t(kMaxSlots) * geometry(n, device, 3).blocks;
 }
 
 } // namespace lgh
 
 extern "C" lgh_status lgh_set_fusion(int enabled) {
     return guard([&] { set_fusion(enabled); });
+}
+
+extern "C" lgh_status lgh_set_tile_bits(int forward, int adjoint) {
+    return guard([&] { set_tile_bits(forward, adjoint); });
 }

@@ -14,16 +14,20 @@ namespace lgh {
 // bits.  A standalone pass holds one gate that the fused kernels cannot run.
 struct Pass {
     bool standalone = false;
-    int reg_bits[4] = {0, 0, 0, 0};
+    int reg_bits[4] = {0, 0, 0, 0}; // the first rb entries are used
     std::vector<int> ops;
 };
 
 void set_fusion(int enabled);
+// register bits of forward and adjoint passes (3 or 4 each)
+void set_tile_bits(int forward, int adjoint);
 int fusion_enabled();
-bool fusable(const KOp &k, int n);
+// rb: register bits of the passes (tile = 2^(6 + rb) amplitudes per wavefront)
+bool fusable(const KOp &k, int n, int rb);
 // trainable_slot[i] >= 0 marks gates whose generator overlap is accumulated
 // (at most 32 per pass); empty for the forward direction.
-std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot);
+std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot,
+                                  int rb);
 int count_passes(const std::vector<KOp> &kops, int n);
 
 // Forward application of a compiled op list through fused passes.

@@ -1,0 +1,455 @@
+// Device form of fused passes: gate descriptors, the register-tiled gate
+// bodies and generator overlaps, shared by the single-state kernels
+// (lgh_fused.hip) and the batched kernels (lgh_batch.hip).  Internal to
+// liblgh.so.
+#pragma once
+
+#include <vector>
+
+#include "lgh_fused.hpp"
+
+namespace lgh {
+namespace fz {
+
+// A wavefront's tile: the 64 lanes carry index bits 0..5, and each lane holds
+// NR = 2^RB registers, one per combination of the pass's RB register bits.
+// RB is a compile-time parameter of the kernels (3 or 4): four register bits
+// fuse more gates per pass, three leave room for two resident wavefronts per
+// SIMD in the adjoint kernel, which holds two states.
+constexpr int kMaxRB = 4;
+constexpr int kMaxNR = 1 << kMaxRB;
+constexpr int kLaneBits = 6;
+constexpr int kMaxSlots = 32;  // trainable gates per pass
+constexpr int kRBForward = 4;  // forward passes (one state)
+constexpr int kRBAdjoint = 3;  // adjoint passes (lambda and one observable state)
+
+// Device description of one fused gate (22 dwords, loaded whole and
+// prefetched one gate ahead).  Controls and parity bits among the register
+// bits become two 16-bit masks over the registers (bit k of rok: the
+// register-bit controls hold in register k; bit k of rpar: odd parity of
+// register k's offset), so each register's test is a compile-time bit of a
+// scalar.  The remaining bits are tested once per lane on the tile base.
+struct FOp {
+    uint32_t meta; // kind | loc << 4 | (slot + 1) << 8 | gkind << 16 | dneg << 20 | mreal << 21
+    uint32_t rbits; // rok | rpar << 16
+    uint64_t cm_rest, pm_rest;
+    uint64_t pad;
+    cx<double> m[4]; // U (dense) or (d_even, d_odd) in m[0], m[1]
+};
+// generator of a trainable gate, indexed by its slot
+struct FGen {
+    cx<double> g[4]; // dense generator, or (g_even, g_odd) in g[0].re, g[1].re
+};
+enum : uint32_t { kLaneDense = 0, kRegDense = 1, kDiag = 2 };
+enum : uint32_t { kGenDense = 0, kGenDiag = 1, kGenX = 2, kGenY = 3 };
+__device__ __forceinline__ uint32_t f_kind(const FOp &o) { return o.meta & 15u; }
+__device__ __forceinline__ int f_loc(const FOp &o) { return int((o.meta >> 4) & 15u); }
+__device__ __forceinline__ int f_slot(const FOp &o) { return int((o.meta >> 8) & 255u) - 1; }
+__device__ __forceinline__ uint32_t f_gkind(const FOp &o) { return (o.meta >> 16) & 15u; }
+__device__ __forceinline__ bool f_dneg(const FOp &o) { return (o.meta >> 20) & 1u; }
+__device__ __forceinline__ bool f_real(const FOp &o) { return (o.meta >> 21) & 1u; }
+
+struct PassDesc {
+    int32_t nops;
+    int32_t nslots;
+    int32_t rb;                  // register bits of the pass
+    int32_t pad;
+    int32_t reg_bits[kMaxRB];
+    int32_t reg_sorted[kMaxRB];  // register bits in ascending order
+    uint64_t reg_off[kMaxNR];    // index offset of each register
+};
+
+// First index of tile t: t's bits go around the lane bits (0..5) and the
+// register bits, which are all >= 6.
+template <int RB> __device__ __forceinline__ uint64_t tile_base(uint64_t t, const int (&rs)[RB]) {
+    uint64_t g = t << kLaneBits;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+        const int p = rs[r];
+        g = ((g >> p) << (p + 1)) | (g & ((1ull << p) - 1ull));
+    }
+    return g;
+}
+
+template <class R> __device__ __forceinline__ cx<R> lo(cx<double> z) { return cx<R>{R(z.re), R(z.im)}; }
+template <class R> __device__ __forceinline__ cx<R> neg(cx<R> a) { return cx<R>{-a.re, -a.im}; }
+
+__device__ __forceinline__ bool reg_ok(const FOp &op, int k) { return (op.rbits >> k) & 1u; }
+__device__ __forceinline__ bool reg_par(const FOp &op, int k) { return (op.rbits >> (16 + k)) & 1u; }
+
+// Dense 2x2 on register bit B.
+template <class R> __device__ __forceinline__ cx<R> rmul(R s, cx<R> a) { return cx<R>{s * a.re, s * a.im}; }
+template <class R> __device__ __forceinline__ cx<R> rfma(R s, cx<R> a, cx<R> c) {
+    return cx<R>{fma(s, a.re, c.re), fma(s, a.im, c.im)};
+}
+template <class R> __device__ __forceinline__ cx<R> sel(bool c, cx<R> x, cx<R> y) {
+    return cx<R>{c ? x.re : y.re, c ? x.im : y.im};
+}
+
+// A gate without controls updates every amplitude of the tile; the masked
+// variants select per register and lane instead of branching.
+// (rok has its bits for registers beyond NR set, so the test is the same for any NR)
+__device__ __forceinline__ bool f_plain(const FOp &o) { return (o.rbits & 0xffffu) == 0xffffu && o.cm_rest == 0; }
+
+// 2x2 matrix in the working precision; Real keeps only the real parts.
+template <class R> struct M2 {
+    cx<R> m00, m01, m10, m11;
+};
+template <class R> __device__ __forceinline__ M2<R> m2(const FOp &op) {
+    return M2<R>{lo<R>(op.m[0]), lo<R>(op.m[1]), lo<R>(op.m[2]), lo<R>(op.m[3])};
+}
+template <class R, bool Real> __device__ __forceinline__ void mat2(const M2<R> &m, cx<R> &x0, cx<R> &x1) {
+    cx<R> y0, y1;
+    if constexpr (Real) {
+        y0 = rfma(m.m01.re, x1, rmul(m.m00.re, x0));
+        y1 = rfma(m.m11.re, x1, rmul(m.m10.re, x0));
+    } else {
+        y0 = cfma(m.m01, x1, cmul(m.m00, x0));
+        y1 = cfma(m.m11, x1, cmul(m.m10, x0));
+    }
+    x0 = y0;
+    x1 = y1;
+}
+// lane bit L <= 3: each lane updates its own amplitude from the partner's
+template <class R, bool Real> __device__ __forceinline__ cx<R> row2(cx<R> cs, cx<R> cp, cx<R> own, cx<R> partner) {
+    if constexpr (Real)
+        return rfma(cp.re, partner, rmul(cs.re, own));
+    else
+        return cfma(cp, partner, cmul(cs, own));
+}
+
+// --- uncontrolled gates -------------------------------------------------
+template <class R, int NR, int B, bool Real> __device__ __forceinline__ void reg_plain(const M2<R> &m, cx<R> (&v)[NR]) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+        if (!(k & (1 << B))) mat2<R, Real>(m, v[k], v[k | (1 << B)]);
+}
+template <class R, int NR, int L, bool Real>
+__device__ __forceinline__ void lane_plain(const M2<R> &m, int lane, cx<R> (&v)[NR]) {
+    if constexpr (L >= 4) {
+#pragma unroll
+        for (int k = 0; k < NR; k += 2) {
+            row_swap_cx<L>(v[k], v[k + 1]);
+            mat2<R, Real>(m, v[k], v[k + 1]);
+            row_swap_cx<L>(v[k], v[k + 1]);
+        }
+    } else {
+        const bool hi = (lane >> L) & 1;
+        const cx<R> cs = hi ? m.m11 : m.m00, cp = hi ? m.m10 : m.m01;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) v[k] = row2<R, Real>(cs, cp, v[k], xor_lane_cx<L>(v[k], lane));
+    }
+}
+
+// --- controlled gates (controls on register bits and/or other bits) ------
+template <class R, int NR, int B>
+__device__ __forceinline__ void reg_masked(const FOp &op, const M2<R> &m, bool lane_ok, cx<R> (&v)[NR]) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if (k & (1 << B)) continue;
+        const int k1 = k | (1 << B);
+        cx<R> x = v[k], y = v[k1];
+        mat2<R, false>(m, x, y);
+        const bool ok = reg_ok(op, k) && lane_ok;
+        v[k] = sel(ok, x, v[k]);
+        v[k1] = sel(ok, y, v[k1]);
+    }
+}
+template <class R, int NR, int L>
+__device__ __forceinline__ void lane_masked(const FOp &op, const M2<R> &m, int lane, bool lane_ok, cx<R> (&v)[NR]) {
+    const bool hi = (lane >> L) & 1;
+    const cx<R> cs = hi ? m.m11 : m.m00, cp = hi ? m.m10 : m.m01;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const cx<R> r = row2<R, false>(cs, cp, v[k], xor_lane_cx<L>(v[k], lane));
+        v[k] = sel(reg_ok(op, k) && lane_ok, r, v[k]);
+    }
+}
+
+template <class R, int NR>
+__device__ __forceinline__ void diag_gate(const FOp &op, uint64_t base, bool lane_ok, cx<R> (&v)[NR]) {
+    if (f_dneg(op)) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) v[k] = sel(reg_ok(op, k) && lane_ok, neg(v[k]), v[k]);
+        return;
+    }
+    const bool lpar = __popcll(base & op.pm_rest) & 1;
+    cx<R> pe = lo<R>(lpar ? op.m[1] : op.m[0]); // phase when the register part is even
+    cx<R> po = lo<R>(lpar ? op.m[0] : op.m[1]);
+    if (!lane_ok) pe = po = cx<R>{R(1), R(0)};
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const cx<R> ph = reg_ok(op, k) ? (reg_par(op, k) ? po : pe) : cx<R>{R(1), R(0)};
+        v[k] = cmul(v[k], ph);
+    }
+}
+
+template <class R, int NR, int B> __device__ __forceinline__ void reg_gate(const FOp &op, bool lane_ok, cx<R> (&v)[NR]) {
+    const M2<R> m = m2<R>(op);
+    if (!f_plain(op))
+        reg_masked<R, NR, B>(op, m, lane_ok, v);
+    else if (f_real(op))
+        reg_plain<R, NR, B, true>(m, v);
+    else
+        reg_plain<R, NR, B, false>(m, v);
+}
+template <class R, int NR, int L>
+__device__ __forceinline__ void lane_gate(const FOp &op, int lane, bool lane_ok, cx<R> (&v)[NR]) {
+    const M2<R> m = m2<R>(op);
+    if (!f_plain(op))
+        lane_masked<R, NR, L>(op, m, lane, lane_ok, v);
+    else if (f_real(op))
+        lane_plain<R, NR, L, true>(m, lane, v);
+    else
+        lane_plain<R, NR, L, false>(m, lane, v);
+}
+
+template <class R, int NR>
+__device__ __forceinline__ void apply_fop(const FOp &op, int lane, uint64_t base, cx<R> (&v)[NR]) {
+    const bool lane_ok = (base & op.cm_rest) == op.cm_rest;
+    const uint32_t kind = f_kind(op);
+    const int loc = f_loc(op);
+    if (kind == kDiag) {
+        diag_gate<R, NR>(op, base, lane_ok, v);
+    } else if (kind == kRegDense) {
+        switch (loc) {
+        case 0: reg_gate<R, NR, 0>(op, lane_ok, v); break;
+        case 1: reg_gate<R, NR, 1>(op, lane_ok, v); break;
+        case 2: reg_gate<R, NR, 2>(op, lane_ok, v); break;
+        default:
+            if constexpr (NR > 8) reg_gate<R, NR, 3>(op, lane_ok, v);
+            break;
+        }
+    } else {
+        switch (loc) {
+        case 0: lane_gate<R, NR, 0>(op, lane, lane_ok, v); break;
+        case 1: lane_gate<R, NR, 1>(op, lane, lane_ok, v); break;
+        case 2: lane_gate<R, NR, 2>(op, lane, lane_ok, v); break;
+        case 3: lane_gate<R, NR, 3>(op, lane, lane_ok, v); break;
+        case 4: lane_gate<R, NR, 4>(op, lane, lane_ok, v); break;
+        default: lane_gate<R, NR, 5>(op, lane, lane_ok, v); break;
+        }
+    }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+template <class R> __device__ __forceinline__ cx<double> up(cx<R> a) { return cx<double>{double(a.re), double(a.im)}; }
+template <class R> __device__ __forceinline__ double im_cdot(cx<R> b, cx<double> q) {
+    return double(b.re) * q.im - double(b.im) * q.re;
+}
+template <class R> __device__ __forceinline__ double re_cdot(cx<R> b, cx<R> a) {
+    return double(b.re) * a.re + double(b.im) * a.im;
+}
+template <class R> __device__ __forceinline__ double im_cdot2(cx<R> b, cx<R> a) {
+    return double(b.re) * a.im - double(b.im) * a.re;
+}
+
+// Im(conj(b0) (G a)_0 + conj(b1) (G a)_1) for a 2x2 generator.  G = X and
+// G = Y avoid the general complex products:
+//   X: Im(conj(b0) a1 + conj(b1) a0);   Y: Re(conj(b1) a0) - Re(conj(b0) a1).
+template <class R>
+__device__ __forceinline__ double pair_dot(uint32_t gk, const FGen &g, cx<R> a0, cx<R> a1, cx<R> b0, cx<R> b1) {
+    if (gk == kGenX) return im_cdot2(b0, a1) + im_cdot2(b1, a0);
+    if (gk == kGenY) return re_cdot(b1, a0) - re_cdot(b0, a1);
+    const cx<double> x0 = up(a0), x1 = up(a1);
+    return im_cdot(b0, cfma(g.g[1], x1, cmul(g.g[0], x0))) + im_cdot(b1, cfma(g.g[3], x1, cmul(g.g[2], x0)));
+}
+// lane bit L <= 3: one lane's row of the pair sum; own = its amplitude,
+// partner = the amplitude in lane ^ 2^L, hi = the lane's bit L
+template <class R>
+__device__ __forceinline__ double row_dot(uint32_t gk, const FGen &g, bool hi, cx<R> own, cx<R> partner, cx<R> b) {
+    if (gk == kGenX) return im_cdot2(b, partner);
+    if (gk == kGenY) return hi ? re_cdot(b, partner) : -re_cdot(b, partner);
+    const cx<double> gs = hi ? g.g[3] : g.g[0], gp = hi ? g.g[2] : g.g[1];
+    return im_cdot(b, cfma(gp, up(partner), cmul(gs, up(own))));
+}
+// diagonal generator: eigenvalue g_even / g_odd by the parity of the mask bits
+template <class R, int NR>
+__device__ __forceinline__ double diag_dot(const FOp &op, const FGen &g, uint64_t base, bool lane_ok,
+                                           const cx<R> (&a)[NR], const cx<R> (&b)[NR]) {
+    const bool lpar = __popcll(base & op.pm_rest) & 1;
+    const double ge = lpar ? g.g[1].re : g.g[0].re, go = lpar ? g.g[0].re : g.g[1].re;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if (!reg_ok(op, k)) continue;
+        s += (reg_par(op, k) ? go : ge) * im_cdot2(b[k], a[k]);
+    }
+    return lane_ok ? s : 0.0;
+}
+
+// Generator overlap of a controlled gate (either state unchanged).
+template <class R, int NR, int B>
+__device__ __forceinline__ double reg_dot_masked(const FOp &op, const FGen &g, bool lane_ok, const cx<R> (&a)[NR],
+                                                 const cx<R> (&b)[NR]) {
+    const uint32_t gk = f_gkind(op);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if (k & (1 << B)) continue;
+        if (!reg_ok(op, k)) continue;
+        const int k1 = k | (1 << B);
+        s += pair_dot<R>(gk, g, a[k], a[k1], b[k], b[k1]);
+    }
+    return lane_ok ? s : 0.0;
+}
+template <class R, int NR, int L>
+__device__ __forceinline__ double lane_dot_masked(const FOp &op, const FGen &g, int lane, bool lane_ok,
+                                                  const cx<R> (&a)[NR], const cx<R> (&b)[NR]) {
+    const bool hi = (lane >> L) & 1;
+    const uint32_t gk = f_gkind(op);
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if (!reg_ok(op, k)) continue;
+        s += row_dot<R>(gk, g, hi, a[k], xor_lane_cx<L>(a[k], lane), b[k]);
+    }
+    return lane_ok ? s : 0.0;
+}
+
+// One reverse step of an uncontrolled dense gate on (lambda, bra): the
+// generator overlap (with Gen) and U^dagger on both states, sharing the lane
+// exchanges between them.
+template <class R, int NR, int B, bool Real, bool Gen>
+__device__ __forceinline__ double adj_reg_plain(const M2<R> &m, uint32_t gk, const FGen *g, cx<R> (&a)[NR],
+                                                cx<R> (&b)[NR]) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if (k & (1 << B)) continue;
+        const int k1 = k | (1 << B);
+        if constexpr (Gen) s += pair_dot<R>(gk, *g, a[k], a[k1], b[k], b[k1]);
+        mat2<R, Real>(m, a[k], a[k1]);
+        mat2<R, Real>(m, b[k], b[k1]);
+    }
+    return s;
+}
+template <class R, int NR, int L, bool Real, bool Gen>
+__device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, const FGen *g, int lane,
+                                                 cx<R> (&a)[NR], cx<R> (&b)[NR]) {
+    double s = 0.0;
+    if constexpr (L >= 4) {
+#pragma unroll
+        for (int k = 0; k < NR; k += 2) {
+            row_swap_cx<L>(a[k], a[k + 1]);
+            row_swap_cx<L>(b[k], b[k + 1]);
+            if constexpr (Gen) s += pair_dot<R>(gk, *g, a[k], a[k + 1], b[k], b[k + 1]);
+            mat2<R, Real>(m, a[k], a[k + 1]);
+            mat2<R, Real>(m, b[k], b[k + 1]);
+            row_swap_cx<L>(a[k], a[k + 1]);
+            row_swap_cx<L>(b[k], b[k + 1]);
+        }
+    } else {
+        const bool hi = (lane >> L) & 1;
+        const cx<R> cs = hi ? m.m11 : m.m00, cp = hi ? m.m10 : m.m01;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const cx<R> pa = xor_lane_cx<L>(a[k], lane);
+            if constexpr (Gen) s += row_dot<R>(gk, *g, hi, a[k], pa, b[k]);
+            a[k] = row2<R, Real>(cs, cp, a[k], pa);
+            b[k] = row2<R, Real>(cs, cp, b[k], xor_lane_cx<L>(b[k], lane));
+        }
+    }
+    return s;
+}
+
+template <class R, int NR, bool Gen>
+__device__ __forceinline__ double adj_plain_dense(const FOp &op, const FGen *g, int lane, cx<R> (&a)[NR],
+                                                  cx<R> (&b)[NR]) {
+    const M2<R> m = m2<R>(op);
+    const uint32_t gk = f_gkind(op);
+    const bool real = f_real(op);
+    const int loc = f_loc(op);
+    if (f_kind(op) == kRegDense) {
+        switch (loc) {
+        case 0: return real ? adj_reg_plain<R, NR, 0, true, Gen>(m, gk, g, a, b) : adj_reg_plain<R, NR, 0, false, Gen>(m, gk, g, a, b);
+        case 1: return real ? adj_reg_plain<R, NR, 1, true, Gen>(m, gk, g, a, b) : adj_reg_plain<R, NR, 1, false, Gen>(m, gk, g, a, b);
+        case 2: return real ? adj_reg_plain<R, NR, 2, true, Gen>(m, gk, g, a, b) : adj_reg_plain<R, NR, 2, false, Gen>(m, gk, g, a, b);
+        default:
+            if constexpr (NR > 8)
+                return real ? adj_reg_plain<R, NR, 3, true, Gen>(m, gk, g, a, b) : adj_reg_plain<R, NR, 3, false, Gen>(m, gk, g, a, b);
+            return 0.0;
+        }
+    }
+    switch (loc) {
+    case 0: return real ? adj_lane_plain<R, NR, 0, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 0, false, Gen>(m, gk, g, lane, a, b);
+    case 1: return real ? adj_lane_plain<R, NR, 1, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 1, false, Gen>(m, gk, g, lane, a, b);
+    case 2: return real ? adj_lane_plain<R, NR, 2, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 2, false, Gen>(m, gk, g, lane, a, b);
+    case 3: return real ? adj_lane_plain<R, NR, 3, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 3, false, Gen>(m, gk, g, lane, a, b);
+    case 4: return real ? adj_lane_plain<R, NR, 4, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 4, false, Gen>(m, gk, g, lane, a, b);
+    default: return real ? adj_lane_plain<R, NR, 5, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 5, false, Gen>(m, gk, g, lane, a, b);
+    }
+}
+
+// Generator overlap before the gate for the general (controlled or
+// diagonal) case.
+template <class R, int NR>
+__device__ __forceinline__ double gen_dot(const FOp &op, const FGen &g, int lane, uint64_t base,
+                                          const cx<R> (&a)[NR], const cx<R> (&b)[NR]) {
+    const bool lane_ok = (base & op.cm_rest) == op.cm_rest;
+    if (f_gkind(op) == kGenDiag) return diag_dot<R, NR>(op, g, base, lane_ok, a, b);
+    const int loc = f_loc(op);
+    if (f_kind(op) == kRegDense) {
+        switch (loc) {
+        case 0: return reg_dot_masked<R, NR, 0>(op, g, lane_ok, a, b);
+        case 1: return reg_dot_masked<R, NR, 1>(op, g, lane_ok, a, b);
+        case 2: return reg_dot_masked<R, NR, 2>(op, g, lane_ok, a, b);
+        default:
+            if constexpr (NR > 8) return reg_dot_masked<R, NR, 3>(op, g, lane_ok, a, b);
+            return 0.0;
+        }
+    }
+    switch (loc) {
+    case 0: return lane_dot_masked<R, NR, 0>(op, g, lane, lane_ok, a, b);
+    case 1: return lane_dot_masked<R, NR, 1>(op, g, lane, lane_ok, a, b);
+    case 2: return lane_dot_masked<R, NR, 2>(op, g, lane, lane_ok, a, b);
+    case 3: return lane_dot_masked<R, NR, 3>(op, g, lane, lane_ok, a, b);
+    case 4: return lane_dot_masked<R, NR, 4>(op, g, lane, lane_ok, a, b);
+    default: return lane_dot_masked<R, NR, 5>(op, g, lane, lane_ok, a, b);
+    }
+}
+
+// Gate descriptors are read through the constant address space: the loads
+// are then scalar, and everything decided from them stays uniform.
+typedef __attribute__((address_space(4))) const FOp CFOp;
+typedef __attribute__((address_space(4))) const FGen CFGen;
+__device__ __forceinline__ FOp load_op(const CFOp *p) {
+    FOp f;
+    f.meta = p->meta;
+    f.rbits = p->rbits;
+    f.cm_rest = p->cm_rest;
+    f.pm_rest = p->pm_rest;
+    f.pad = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) f.m[q] = cx<double>{p->m[q].re, p->m[q].im};
+    return f;
+}
+__device__ __forceinline__ FGen load_gen(const CFGen *p) {
+    FGen g;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) g.g[q] = cx<double>{p->g[q].re, p->g[q].im};
+    return g;
+}
+
+// Host: device description of one pass (descriptor, gates, and for the
+// reverse direction the generator of every trainable gate by slot).
+struct Built {
+    PassDesc pd;
+    std::vector<FOp> ops;
+    std::vector<FGen> gens; // one per slot
+};
+
+// reverse: gates in reverse order (the adjoint sweep); gens / slot_of mark
+// trainable gates (nullptr in the forward direction, which instead merges
+// consecutive single-qubit gates on one bit unless merge is false; without
+// merging, ops[j] is the j-th gate of the pass in execution order).
+Built build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse, const std::vector<KGen> *gens,
+                 const std::vector<int> *slot_of, int rb, bool merge = true);
+
+} // namespace fz
+} // namespace lgh

@@ -110,6 +110,78 @@ template <class R> __device__ __forceinline__ cx<R> shfl_xor_cx(cx<R> a, int mas
     return cx<R>{__shfl_xor(a.re, mask, 64), __shfl_xor(a.im, mask, 64)};
 }
 
+// Value of lane (lane ^ 2^L) without going through LDS: DPP quad
+// permutations for L = 0, 1, two DPP row shifts for L = 2, 3 (row_shl:N reads
+// lane + N, row_shr:N reads lane - N inside a 16-lane row), and the gfx950
+// permlane swaps for L = 4, 5 (odd 16- or 32-lane rows of the first operand
+// trade places with even rows of the second).
+template <int L> __device__ __forceinline__ int xor_lane_b32(int v, int lane) {
+    static_assert(L >= 0 && L < 6, "lane bit");
+    if constexpr (L == 0) {
+        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false); // quad_perm [1,0,3,2]
+    } else if constexpr (L == 1) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false); // quad_perm [2,3,0,1]
+    } else if constexpr (L == 2 || L == 3) {
+        constexpr int n = 1 << L;
+        const int up = __builtin_amdgcn_update_dpp(0, v, 0x100 + n, 0xF, 0xF, false); // row_shl:n
+        const int dn = __builtin_amdgcn_update_dpp(0, v, 0x110 + n, 0xF, 0xF, false); // row_shr:n
+        return (lane & n) ? dn : up;
+    } else if constexpr (L == 4) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return int((lane & 16) ? p[0] : p[1]);
+    } else {
+        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return int((lane & 32) ? p[0] : p[1]);
+    }
+}
+template <int L> __device__ __forceinline__ float xor_lane(float v, int lane) {
+    return __int_as_float(xor_lane_b32<L>(__float_as_int(v), lane));
+}
+template <int L> __device__ __forceinline__ double xor_lane(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const int lo = xor_lane_b32<L>(int(b & 0xffffffffll), lane);
+    const int hi = xor_lane_b32<L>(int(b >> 32), lane);
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+template <int L, class R> __device__ __forceinline__ cx<R> xor_lane_cx(cx<R> a, int lane) {
+    return cx<R>{xor_lane<L>(a.re, lane), xor_lane<L>(a.im, lane)};
+}
+
+// Row swap for lane bit L = 4 or 5 between two registers x, y: afterwards
+// every lane holds in (x, y) the two amplitudes that differ only in lane
+// bit L (x: bit clear, y: bit set), taken from x in the lanes with the bit
+// clear and from y in the others.  Applying it again restores the layout.
+template <int L> __device__ __forceinline__ void row_swap_b32(int &x, int &y) {
+    static_assert(L == 4 || L == 5, "row swaps exist for lane bits 4 and 5");
+    if constexpr (L == 4) {
+        const auto p = __builtin_amdgcn_permlane16_swap(x, y, false, false);
+        x = int(p[0]);
+        y = int(p[1]);
+    } else {
+        const auto p = __builtin_amdgcn_permlane32_swap(x, y, false, false);
+        x = int(p[0]);
+        y = int(p[1]);
+    }
+}
+template <int L> __device__ __forceinline__ void row_swap(float &x, float &y) {
+    int a = __float_as_int(x), b = __float_as_int(y);
+    row_swap_b32<L>(a, b);
+    x = __int_as_float(a);
+    y = __int_as_float(b);
+}
+template <int L> __device__ __forceinline__ void row_swap(double &x, double &y) {
+    const long long bx = __double_as_longlong(x), by = __double_as_longlong(y);
+    int xl = int(bx & 0xffffffffll), xh = int(bx >> 32), yl = int(by & 0xffffffffll), yh = int(by >> 32);
+    row_swap_b32<L>(xl, yl);
+    row_swap_b32<L>(xh, yh);
+    x = __longlong_as_double((static_cast<long long>(xh) << 32) | static_cast<unsigned>(xl));
+    y = __longlong_as_double((static_cast<long long>(yh) << 32) | static_cast<unsigned>(yl));
+}
+template <int L, class R> __device__ __forceinline__ void row_swap_cx(cx<R> &x, cx<R> &y) {
+    row_swap<L>(x.re, y.re);
+    row_swap<L>(x.im, y.im);
+}
+
 constexpr int kMaxFixed = 16; // target + control bits that a kernel enumerates around
 constexpr int kMaxRegK = 4;   // dense gates held in registers up to 4 targets
 constexpr int kMaxLdsK = 13;  // dense gates staged through LDS up to 13 targets
@@ -176,6 +248,18 @@ struct DeviceGuard {
 };
 
 int sm_count(int device);
+
+// Optional launch timing (lgh_profile_enable): HIP events recorded on the
+// launch stream around each launch of the hot kernels, summed per kind by
+// lgh_profile_read.
+enum ProfKind { kProfFusedPass = 0, kProfFusedAdj = 1, kProfBatch = 2, kProfKinds = 3 };
+struct ProfScope {
+    int kind;
+    hipStream_t stream;
+    hipEvent_t start = nullptr;
+    ProfScope(int kind, hipStream_t stream);
+    ~ProfScope();
+};
 
 } // namespace lgh
 

@@ -6,6 +6,7 @@
 // DevicePool.hpp:15-128), StateVectorCudaBase (StateVectorCudaBase.hpp:54-313)
 // and initSV.cu:64-117.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <map>
 #include <unordered_map>
@@ -36,6 +37,40 @@ int sm_count(int device) {
     LGH_HIP(hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, device));
     cache[device] = v;
     return v;
+}
+
+namespace {
+std::atomic<int> g_prof{0};
+struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+std::vector<ProfRec> g_prof_recs;
+} // namespace
+
+ProfScope::ProfScope(int k, hipStream_t s) : kind(k), stream(s) {
+    if (!g_prof.load()) return;
+    if (hipEventCreate(&start) != hipSuccess) {
+        start = nullptr;
+        return;
+    }
+    if (hipEventRecord(start, stream) != hipSuccess) {
+        (void)hipEventDestroy(start);
+        start = nullptr;
+    }
+}
+
+ProfScope::~ProfScope() {
+    if (!start) return;
+    hipEvent_t stop = nullptr;
+    if (hipEventCreate(&stop) != hipSuccess || hipEventRecord(stop, stream) != hipSuccess) {
+        if (stop) (void)hipEventDestroy(stop);
+        (void)hipEventDestroy(start);
+        return;
+    }
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_recs.push_back(ProfRec{kind, start, stop});
 }
 
 Fixed make_fixed(std::vector<int> bits) {
@@ -531,3 +566,36 @@ lgh_status lgh_sv_synchronize(const lgh_sv *sv) {
 }
 
 } // extern "C"
+
+extern "C" lgh_status lgh_profile_enable(int enabled) {
+    return guard([&] {
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        for (ProfRec &r : g_prof_recs) {
+            (void)hipEventSynchronize(r.b);
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        g_prof_recs.clear();
+        g_prof.store(enabled ? 1 : 0);
+    });
+}
+
+extern "C" lgh_status lgh_profile_read(int kind, int64_t *launches, double *total_ms) {
+    return guard([&] {
+        LGH_REQUIRE(kind >= 0 && kind < kProfKinds, "unknown profile kind");
+        LGH_REQUIRE(launches && total_ms, "null pointer");
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        int64_t n = 0;
+        double t = 0.0;
+        for (ProfRec &r : g_prof_recs) {
+            if (r.kind != kind) continue;
+            LGH_HIP(hipEventSynchronize(r.b));
+            float ms = 0.f;
+            LGH_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            ++n;
+            t += ms;
+        }
+        *launches = n;
+        *total_ms = t;
+    });
+}

@@ -58,6 +58,12 @@ class Op(C.Structure):
 _lib = None
 
 
+class Request(C.Structure):
+    _fields_ = [("num_qubits", C.c_int32), ("ops", C.POINTER(Op)), ("num_ops", C.c_int32),
+                ("obs_paulis", C.POINTER(C.c_int32)), ("obs_wires", C.POINTER(C.c_int32)), ("obs_len", C.c_int32),
+                ("want_grad", C.c_int32)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -103,6 +109,21 @@ def lib():
         L.lgh_stream_create.argtypes = [C.c_int, C.POINTER(vp)]
         L.lgh_stream_destroy.argtypes = [vp]
         L.lgh_stream_synchronize.argtypes = [vp]
+        L.lgh_set_tile_bits.argtypes = [C.c_int, C.c_int]
+        L.lgh_profile_enable.argtypes = [C.c_int]
+        L.lgh_profile_read.argtypes = [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        L.lgh_tmpl_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Op), C.c_int32, vp, vp, vp, C.c_int32,
+                                      C.c_int32]
+        L.lgh_tmpl_destroy.argtypes = [vp]
+        L.lgh_tmpl_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.lgh_batch_run.argtypes = [vp, C.c_int32, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.lgh_batch_run_serial.argtypes = [vp, C.c_int32, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.lgh_server_create.argtypes = [C.POINTER(vp), C.c_int32, C.c_uint64, C.c_int32, C.c_int32]
+        L.lgh_server_destroy.argtypes = [vp]
+        L.lgh_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
+        L.lgh_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double), vp, C.c_int64, vp]
+        L.lgh_server_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                       C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -115,6 +136,11 @@ def check(rc):
 def set_fusion(enabled: bool):
     """Switch gate fusion of lgh_apply sequences and the adjoint sweep."""
     check(lib().lgh_set_fusion(int(bool(enabled))))
+
+
+def set_tile_bits(forward: int, adjoint: int):
+    """Register bits (3 or 4) of the fused forward and adjoint passes."""
+    check(lib().lgh_set_tile_bits(int(forward), int(adjoint)))
 
 
 def device_count() -> int:
@@ -334,3 +360,137 @@ def adjoint_jacobian(psi: State, bras, ops, trainable, apply_operations=False) -
     check(lib().lgh_adjoint_jacobian(psi.handle, hs, len(bras), arr, len(ops), tp.ctypes.data, len(trainable),
                                      int(apply_operations), jac.ctypes.data))
     return jac[: len(bras) * len(trainable)].reshape(len(bras), len(trainable))
+
+
+# ---------------------------------------------------------------- launch timing
+
+PROFILE_KINDS = {"fused_pass": 0, "fused_adjoint": 1, "batch": 2}
+
+
+def profile_enable(enabled: bool = True):
+    """Record HIP events around every launch of the hot kernels (clears what was recorded)."""
+    check(lib().lgh_profile_enable(int(bool(enabled))))
+
+
+def profile_read(kind: str):
+    """(launches, total milliseconds) recorded for ``kind`` since profiling was enabled."""
+    n, t = C.c_int64(), C.c_double()
+    check(lib().lgh_profile_read(PROFILE_KINDS[kind], C.byref(n), C.byref(t)))
+    return n.value, t.value
+
+
+# ---------------------------------------------------------------- batched circuits
+
+PAULI_GATE = {"X": GATE_ID["PauliX"], "Y": GATE_ID["PauliY"], "Z": GATE_ID["PauliZ"]}
+
+
+class Template:
+    """Gate structure shared by circuits that differ only in their parameters.
+
+    ``ops``: ``(name, wires, params, inverse)`` tuples; ``param_index[i]`` is the
+    circuit parameter driving op i (``-1``: fixed as given).  ``observable``:
+    Pauli word as ``[("Z", wire), ...]``.
+    """
+
+    def __init__(self, num_qubits, ops, param_index, observable, device=0):
+        arr, keep = make_ops(ops)
+        pidx = np.ascontiguousarray(list(param_index) or [0], dtype=np.int32)
+        obs_p = np.ascontiguousarray([PAULI_GATE[p] for p, _ in observable] or [0], dtype=np.int32)
+        obs_w = np.ascontiguousarray([int(w) for _, w in observable] or [0], dtype=np.int32)
+        self.handle = C.c_void_p()
+        check(lib().lgh_tmpl_create(C.byref(self.handle), int(num_qubits), arr, len(ops), pidx.ctypes.data,
+                                    obs_p.ctypes.data, obs_w.ctypes.data, len(observable), int(device)))
+        npar, nq = C.c_int32(), C.c_int32()
+        check(lib().lgh_tmpl_info(self.handle, C.byref(npar), C.byref(nq)))
+        self.num_params, self.num_qubits = npar.value, nq.value
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def close(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            lib().lgh_tmpl_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+
+def batch_run(templates, params, grad=True, dtype=np.complex128, serial=False, stream=None):
+    """Run circuit c = ``templates[c]`` with ``params[c]`` (a 1-D array per circuit).
+
+    Returns ``(expvals, jacobians)``: expvals[c] = <P>, jacobians[c] = d<P>/d params[c]
+    (None without ``grad``).
+    """
+    nc = len(templates)
+    hs = (C.c_void_p * max(1, nc))(*[t.handle.value for t in templates])
+    flat = [np.asarray(p, dtype=np.float64).ravel() for p in params]
+    for t, p in zip(templates, flat):
+        if p.size != t.num_params:
+            raise ValueError(f"template takes {t.num_params} parameters, got {p.size}")
+    offs = np.zeros(max(1, nc), dtype=np.int64)
+    if nc:
+        offs[1:nc] = np.cumsum([p.size for p in flat])[:-1]
+    allp = np.ascontiguousarray(np.concatenate(flat) if flat and sum(p.size for p in flat) else np.zeros(1))
+    ev = np.zeros(max(1, nc))
+    jac = np.zeros(max(1, int(sum(p.size for p in flat)))) if grad else None
+    fn = lib().lgh_batch_run_serial if serial else lib().lgh_batch_run
+    code = LGH_C128 if np.dtype(dtype) == np.complex128 else LGH_C64
+    check(fn(hs, nc, code, allp.ctypes.data, offs.ctypes.data, ev.ctypes.data,
+             jac.ctypes.data if grad else None, offs.ctypes.data if grad else None, stream))
+    jacs = [jac[o:o + p.size].copy() for o, p in zip(offs[:nc], flat)] if grad else None
+    return ev[:nc].copy(), jacs
+
+
+# ---------------------------------------------------------------- serving
+
+class Server:
+    """Request queue in front of one device (lgh_server_*): continuous
+    batching of small circuits, one state per large circuit, memory admission.
+
+    ``submit(n, ops, observable, grad)`` returns a ticket; ``wait(ticket)``
+    returns ``(expval, grad or None, (queued_ms, run_ms, total_ms))``.
+    """
+
+    def __init__(self, device=0, memory_budget=0, num_streams=4, batch_max_qubits=20):
+        self.handle = C.c_void_p()
+        check(lib().lgh_server_create(C.byref(self.handle), int(device), int(memory_budget), int(num_streams),
+                                      int(batch_max_qubits)))
+        self._npar = {}
+
+    def submit(self, num_qubits, ops, observable, grad=False) -> int:
+        arr, keep = make_ops(ops)
+        obs_p = np.ascontiguousarray([PAULI_GATE[p] for p, _ in observable] or [0], dtype=np.int32)
+        obs_w = np.ascontiguousarray([int(w) for _, w in observable] or [0], dtype=np.int32)
+        req = Request(int(num_qubits), C.cast(arr, C.POINTER(Op)), len(ops),
+                      obs_p.ctypes.data_as(C.POINTER(C.c_int32)), obs_w.ctypes.data_as(C.POINTER(C.c_int32)),
+                      len(observable), int(bool(grad)))
+        t = C.c_uint64()
+        check(lib().lgh_server_submit(self.handle, C.byref(req), C.byref(t)))
+        self._npar[t.value] = (sum(1 for o in ops if len(o[2]) > 0), bool(grad))
+        return t.value
+
+    def wait(self, ticket):
+        npar, grad = self._npar.pop(ticket)
+        e = C.c_double()
+        g = np.zeros(max(1, npar))
+        tm = np.zeros(3)
+        check(lib().lgh_server_wait(self.handle, int(ticket), C.byref(e), g.ctypes.data, g.size, tm.ctypes.data))
+        return e.value, (g[:npar].copy() if grad else None), tuple(tm)
+
+    def stats(self):
+        v = [C.c_uint64() for _ in range(4)]
+        check(lib().lgh_server_stats(self.handle, *[C.byref(x) for x in v]))
+        return {"batches": v[0].value, "batched_requests": v[1].value, "single_requests": v[2].value,
+                "budget": v[3].value}
+
+    def close(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            check(lib().lgh_server_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -98,6 +98,24 @@ def tfim_csr(n: int, J: float = 1.0, h: float = 0.7):
     return indptr, cols.ravel(), vals.ravel()
 
 
+def ring_template(n: int, layers: int = 3):
+    """Gate structure of the batched workload's circuits: per layer RX, RY, RZ
+    on every wire, then a CNOT ring.  Returns (ops, param_index) with one
+    parameter per rotation: 12n gates and 9n parameters for 3 layers."""
+    ops, pidx, k = [], [], 0
+    for _ in range(layers):
+        for w in range(n):
+            for g in ("RX", "RY", "RZ"):
+                ops.append((g, [w], [0.0], False))
+                pidx.append(k)
+                k += 1
+        if n > 1:
+            for w in range(n):
+                ops.append(("CNOT", [w, (w + 1) % n], [], False))
+                pidx.append(-1)
+    return ops, pidx
+
+
 def batch_widths(count: int = 4096, lo: int = 8, hi: int = 20, seed: int = 1337):
     rng = np.random.default_rng(seed)
     return [int(x) for x in rng.integers(lo, hi + 1, count)]

@@ -136,3 +136,30 @@ def test_state_creation_without_device_raises():
         m.LightningGPU_C128(3)
     with pytest.raises(N.LghError):
         N.State(3)
+
+
+def test_batch_template_validation_without_device():
+    """Template compilation is host work: structure checks and parameter
+    counts need no GPU."""
+    from pennylane_lightning_gpu.workloads import ring_template
+
+    for n in (8, 12, 20):
+        ops, pidx = ring_template(n)
+        t = N.Template(n, ops, pidx, [("Z", 0)])
+        assert (t.num_params, t.num_qubits) == (9 * n, n)
+    ops = [("RX", [0], [0.0], False), ("CRY", [0, 1], [0.0], True), ("RX", [1], [0.0], False)]
+    assert N.Template(2, ops, [0, 1, 0], [("Y", 1)]).num_params == 2
+    bad = [
+        (21, [("RX", [0], [0.1], False)], [0], [("Z", 0)], "1 to 20 qubits"),
+        (4, [("SWAP", [0, 1], [], False)], [-1], [("Z", 0)], "not supported in a batched circuit"),
+        (4, [("Hadamard", [0], [], False)], [0], [("Z", 0)], "cannot take a circuit parameter"),
+        (4, [("RX", [5], [0.1], False)], [0], [("Z", 0)], "wire index out of range"),
+        (4, [("RX", [0], [0.1], False)], [0], [("Z", 7)], "observable wire out of range"),
+        (4, [("RX", [0], [0.1], False)], [0], [("Z", 1), ("X", 1)], "repeated observable wire"),
+        (12, [("RX", [0], [0.1], False)], [0], [("X", w) for w in range(6)], "more than one pass"),
+    ]
+    for n, ops, pidx, obs, msg in bad:
+        with pytest.raises(N.LghError, match=msg):
+            N.Template(n, ops, pidx, obs)
+    with pytest.raises(ValueError):
+        N.batch_run([N.Template(2, ops, [0, 1, 0], [("Z", 0)])], [np.zeros(3)])

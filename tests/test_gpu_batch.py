@@ -1,0 +1,164 @@
+"""Batched circuits (lgh_tmpl_create / lgh_batch_run) against the CPU oracle.
+
+Each circuit is a template's gates with the circuit's own parameters; the
+oracle runs the same gates on |0..0>, takes <P> and its adjoint Jacobian,
+and sums the Jacobian columns of gates that share a parameter.  The serial
+form (one state and one device call sequence per circuit) must agree with the
+persistent batched kernel, and central finite differences (h = 1e-4, error
+O(h^2) ~ 1e-8) pin the gradient independently of the oracle's adjoint.
+"""
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+from _tol import close128, close64
+
+pytestmark = pytest.mark.gpu
+
+N = pytest.importorskip("pennylane_lightning_gpu._native")
+
+
+def ring_template(n, layers=3):
+    """Per layer RX, RY, RZ on every wire, then a CNOT ring (the batched
+    workload's circuit family): 12n gates, 9n parameters for 3 layers."""
+    ops, pidx, k = [], [], 0
+    for _ in range(layers):
+        for w in range(n):
+            for g in ("RX", "RY", "RZ"):
+                ops.append((g, [w], [0.0], False))
+                pidx.append(k)
+                k += 1
+        for w in range(n):
+            ops.append(("CNOT", [w, (w + 1) % n], [], False))
+            pidx.append(-1)
+    return ops, pidx
+
+
+def bind(ops, pidx, theta):
+    return [(o[0], o[1], [theta[p]] if p >= 0 else o[2], o[3]) for o, p in zip(ops, pidx)]
+
+
+def oracle_run(n, ops, pidx, theta, obs):
+    """<P> and d<P>/d theta through the oracle."""
+    bound = bind(ops, pidx, theta)
+    psi = po.apply_ops(n, po.zero_state(n), bound)
+    pops = [("Pauli" + p, [w], [], False) for p, w in obs]
+    bra = po.apply_ops(n, psi, pops)
+    e = float(np.real(np.vdot(psi, bra)))
+    par_ops = [i for i, o in enumerate(bound) if o[2]]
+    tr = [j for j, i in enumerate(par_ops) if pidx[i] >= 0]
+    g = np.zeros(max(pidx) + 1 if max(pidx) >= 0 else 0)
+    if not tr:
+        return e, g
+    jac_ops = po.adjoint_jacobian(n, psi, [bra], bound, tr)[0]
+    for col, j in enumerate(tr):
+        g[pidx[par_ops[j]]] += jac_ops[col]
+    return e, g
+
+
+@pytest.mark.parametrize("n", [3, 8, 10, 11, 13])
+def test_batch_ring_matches_oracle(n):
+    rng = np.random.default_rng(100 + n)
+    ops, pidx = ring_template(n)
+    obs = [("Z", 0)]
+    t = N.Template(n, ops, pidx, obs)
+    thetas = [rng.uniform(0, 2 * np.pi, t.num_params) for _ in range(3)]
+    ev, jac = N.batch_run([t] * 3, thetas)
+    for c in range(3):
+        e, g = oracle_run(n, ops, pidx, thetas[c], obs)
+        close128(ev[c], e)
+        close128(jac[c], g)
+
+
+def test_batch_c64_matches_oracle():
+    rng = np.random.default_rng(7)
+    n = 12
+    ops, pidx = ring_template(n, layers=2)
+    obs = [("Z", 0), ("Z", n - 1)]
+    t = N.Template(n, ops, pidx, obs)
+    theta = rng.uniform(0, 2 * np.pi, t.num_params)
+    ev, jac = N.batch_run([t], [theta], dtype=np.complex64)
+    e, g = oracle_run(n, ops, pidx, theta, obs)
+    close64(ev[0], e)
+    close64(jac[0], g)
+
+
+def test_batch_mixed_gates_observable_and_sharing():
+    """Controlled and diagonal parametric gates, inverse flags, fixed gates,
+    a shared parameter and an X/Y/Z observable."""
+    rng = np.random.default_rng(11)
+    n = 11
+    ops = [("Hadamard", [0], [], False), ("RX", [1], [0.0], False), ("CRY", [1, 4], [0.0], False),
+           ("CRZ", [4, 10], [0.0], True), ("PhaseShift", [2], [0.0], False), ("S", [3], [], False),
+           ("ControlledPhaseShift", [2, 9], [0.0], False), ("IsingZZ", [5, 7], [0.0], False),
+           ("MultiRZ", [0, 6, 8], [0.0], True), ("CRX", [8, 3], [0.0], False), ("Toffoli", [0, 1, 2], [], False),
+           ("T", [9], [], True), ("RY", [10], [0.0], True), ("CZ", [6, 10], [], False), ("RX", [5], [0.0], False),
+           ("RY", [0], [0.37], False), ("PauliY", [7], [], False), ("RZ", [7], [0.0], False)]
+    # RX on wire 1 and wire 5 share parameter 0; RY(0.37) on wire 0 stays fixed
+    pidx = [-1, 0, 1, 2, 3, -1, 4, 5, 6, 7, -1, -1, 8, -1, 0, -1, -1, 9]
+    obs = [("X", 1), ("Y", 4), ("Z", 10)]
+    t = N.Template(n, ops, pidx, obs)
+    assert t.num_params == 10
+    thetas = [rng.uniform(0, 2 * np.pi, 10) for _ in range(2)]
+    ev, jac = N.batch_run([t, t], thetas)
+    for c in range(2):
+        e, g = oracle_run(n, ops, pidx, thetas[c], obs)
+        close128(ev[c], e)
+        close128(jac[c], g)
+
+
+def test_batch_matches_serial_and_mixes_templates():
+    """Uneven circuits of several templates in one launch equal the
+    one-circuit-at-a-time form, and each circuit's result does not depend on
+    what else is in the batch."""
+    rng = np.random.default_rng(5)
+    sizes = [8, 14, 10, 12, 9, 14, 11, 13]
+    tm = {n: N.Template(n, *ring_template(n), [("Z", 0)]) for n in set(sizes)}
+    ts = [tm[n] for n in sizes * 3]
+    thetas = [rng.uniform(0, 2 * np.pi, t.num_params) for t in ts]
+    ev, jac = N.batch_run(ts, thetas)
+    ev_s, jac_s = N.batch_run(ts, thetas, serial=True)
+    close128(ev, ev_s)
+    for a, b in zip(jac, jac_s):
+        close128(a, b)
+    # alone: the first circuit by itself
+    ev1, jac1 = N.batch_run(ts[:1], thetas[:1])
+    close128(ev1[0], ev[0])
+    close128(jac1[0], jac[0])
+
+
+def test_batch_gradient_central_differences():
+    rng = np.random.default_rng(3)
+    n = 10
+    ops, pidx = ring_template(n, layers=2)
+    t = N.Template(n, ops, pidx, [("Z", 0)])
+    theta = rng.uniform(0, 2 * np.pi, t.num_params)
+    _, jac = N.batch_run([t], [theta])
+    h = 1e-4
+    pick = rng.choice(t.num_params, size=8, replace=False)
+    plus, minus = [], []
+    for j in pick:
+        tp, tm_ = theta.copy(), theta.copy()
+        tp[j] += h
+        tm_[j] -= h
+        plus.append(tp)
+        minus.append(tm_)
+    ep, _ = N.batch_run([t] * len(pick), plus, grad=False)
+    em, _ = N.batch_run([t] * len(pick), minus, grad=False)
+    fd = (ep - em) / (2 * h)
+    assert np.max(np.abs(fd - jac[0][pick])) < 1e-7
+
+
+def test_batch_without_gradient_and_fixed_only():
+    n = 10
+    ops = [("Hadamard", [w], [], False) for w in range(n)] + [("CZ", [0, 1], [], False)]
+    # <X5 X7> = 1 exactly (wires outside the CZ stay |+>); <X0> = 0 exactly
+    t = N.Template(n, ops, [-1] * len(ops), [("X", 5), ("X", 7)])
+    assert t.num_params == 0
+    ev, jac = N.batch_run([t, t], [np.zeros(0), np.zeros(0)])
+    e, _ = oracle_run(n, ops, [-1] * len(ops), np.zeros(0), [("X", 5), ("X", 7)])
+    close128(ev, [e, e])
+    t0 = N.Template(n, ops, [-1] * len(ops), [("X", 0)])
+    ev0, _ = N.batch_run([t0], [np.zeros(0)], grad=False)
+    assert abs(ev0[0]) < 1e-14
+    assert jac == [] or all(j.size == 0 for j in jac)

@@ -409,3 +409,28 @@ def test_fused_adjoint_config2_shape():
     ref = po.adjoint_jacobian(n, psi, [bra], ops, tp)
     got = N.adjoint_jacobian(N.State(n).reset().apply(ops), [N.State(n).upload(bra)], ops, tp)
     close128(got, ref)
+
+
+@pytest.mark.parametrize("bits", [(3, 4), (4, 4), (3, 3)])
+def test_fused_tile_bits_variants(bits):
+    """Every tile height of the fused forward and adjoint kernels (the
+    defaults are 4 forward, 3 adjoint) against the oracle."""
+    rng = np.random.default_rng(977)
+    n = 12
+    ops = [o for o in mixed_circuit(n, rng, layers=2) if len(o[2]) <= 1]
+    init = rand_state(n, rng)
+    psi = po.apply_ops(n, init, ops)
+    bras = obs_states(n, psi, [[("PauliZ", [0])], [("PauliX", [n - 1]), ("PauliY", [3])]])
+    npar = sum(1 for o in ops if o[2])
+    tp = list(range(0, npar, 2))
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    try:
+        N.set_tile_bits(*bits)
+        close128(N.State(n).upload(init).apply(ops).download(), psi)
+        got = N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp)
+        close128(got, ref)
+        got64 = N.adjoint_jacobian(N.State(n, np.complex64).upload(psi.astype(np.complex64)),
+                                   [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
+        close64(got64, ref)
+    finally:
+        N.set_tile_bits(4, 3)
