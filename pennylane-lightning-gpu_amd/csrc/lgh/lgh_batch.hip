@@ -148,7 +148,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_prep(const CircDev *__restrict
             }
             bool real = (f.meta & 15u) != kDiag;
             for (int q = 0; q < 4; ++q) real = real && f.m[q].im == 0.0;
-            f.meta = (f.meta & ~(1u << 21)) | (uint32_t(real) << 21);
+            const bool dsym = (f.meta & 15u) == kDiag && f.m[1].re == f.m[0].re && f.m[1].im == -f.m[0].im;
+            f.meta = (f.meta & ~(3u << 21)) | (uint32_t(real) << 21) | (uint32_t(dsym) << 22);
         }
         cd.fops[i] = f;
     }
@@ -254,6 +255,13 @@ __device__ __forceinline__ void rev_pass(const CTPass *tp, const CFOp *fops, con
                         acc[slot][tid] += adj_plain_dense<R, NR, true>(cur, &gg, lane, a, b);
                     } else {
                         adj_plain_dense<R, NR, false>(cur, nullptr, lane, a, b);
+                    }
+                } else if (diag_fast(cur)) {
+                    if (slot >= 0) {
+                        const CFGen *gp = gens + slot;
+                        acc[slot][tid] += adj_diag_step<R, NR, true>(cur, gp->g[0].re, gp->g[1].re, base, a, b);
+                    } else {
+                        adj_diag_step<R, NR, false>(cur, 0.0, 0.0, base, a, b);
                     }
                 } else {
                     if (slot >= 0) {
@@ -534,7 +542,8 @@ void batch_run_t(lgh_tmpl *const *tmpls, int32_t ncirc, const double *params, co
     hipLaunchKernelGGL(k_batch_prep, dim3(unsigned(ncirc)), dim3(kBlock), 0, ctx->stream,
                        reinterpret_cast<const CircDev *>(d), d_cnt);
     LGH_LAUNCH_CHECK();
-    const unsigned grid = unsigned(std::min<int64_t>(ncirc, int64_t(std::max(1, sm_count(dev)))));
+    // two resident workgroups per CU (the kernel runs at two waves per SIMD)
+    const unsigned grid = unsigned(std::min<int64_t>(ncirc, 2 * int64_t(std::max(1, sm_count(dev)))));
     {
         ProfScope prof(kProfBatch, ctx->stream);
         hipLaunchKernelGGL(k_batch<R>, dim3(grid), dim3(kBlock), 0, ctx->stream,

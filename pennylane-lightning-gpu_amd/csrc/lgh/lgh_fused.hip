@@ -138,17 +138,50 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
             release(i);
             continue;
         }
-        // register bits: targets of the earliest ready dense gates outside the lane bits
+        // Register bits are chosen while the pass grows: absorb every ready
+        // gate whose targets are lane or register bits; when none is left
+        // and a register bit is free, add the target of the earliest ready
+        // dense gate that is blocked, and absorb again.  A chain of gates
+        // (each one's target the next one's control) then fills a pass
+        // instead of starting a new one per gate.
         Pass p;
         uint64_t reg = 0;
         int nreg = 0;
-        for (int i : ready) {
-            if (!fusable(ops[i], n, rb) || ops[i].kind != KOp::DENSE) continue;
-            const uint64_t t = use[i].nondiag & ~lane_mask;
-            if (t && !(reg & t) && nreg < rb) {
-                reg |= t;
-                ++nreg;
+        int slots = 0;
+        auto trainable = [&](int i) { return !trainable_slot.empty() && trainable_slot[i] >= 0; };
+        for (;;) {
+            bool progress = true;
+            while (progress) {
+                progress = false;
+                std::sort(ready.begin(), ready.end());
+                for (size_t r = 0; r < ready.size(); ++r) {
+                    const int i = ready[r];
+                    if (!fusable(ops[i], n, rb)) continue;
+                    const uint64_t t = use[i].nondiag & ~lane_mask;
+                    if (t & ~reg) continue;
+                    if (trainable(i) && slots >= kMaxSlots) continue;
+                    slots += trainable(i);
+                    p.ops.push_back(i);
+                    ready.erase(ready.begin() + r);
+                    release(i);
+                    progress = true;
+                    break;
+                }
             }
+            if (nreg >= rb) break;
+            uint64_t grow = 0;
+            for (int i : ready) { // sorted by the absorption loop
+                if (!fusable(ops[i], n, rb) || ops[i].kind != KOp::DENSE) continue;
+                if (trainable(i) && slots >= kMaxSlots) continue;
+                const uint64_t t = use[i].nondiag & ~lane_mask & ~reg;
+                if (t) {
+                    grow = t;
+                    break;
+                }
+            }
+            if (!grow) break;
+            reg |= grow;
+            nreg += __builtin_popcountll(grow);
         }
         // pad with the lowest unused bits so that every pass has rb register bits
         for (int b = kLaneBits; b < n && nreg < rb; ++b)
@@ -156,26 +189,6 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
                 reg |= 1ull << b;
                 ++nreg;
             }
-        int slots = 0;
-        bool progress = true;
-        while (progress) {
-            progress = false;
-            std::sort(ready.begin(), ready.end());
-            for (size_t r = 0; r < ready.size(); ++r) {
-                const int i = ready[r];
-                if (!fusable(ops[i], n, rb)) continue;
-                const uint64_t t = use[i].nondiag & ~lane_mask;
-                if (t & ~reg) continue;
-                const bool tr = !trainable_slot.empty() && trainable_slot[i] >= 0;
-                if (tr && slots >= kMaxSlots) continue;
-                slots += tr;
-                p.ops.push_back(i);
-                ready.erase(ready.begin() + r);
-                release(i);
-                progress = true;
-                break;
-            }
-        }
         for (int b = 0, k = 0; b < n; ++b)
             if (reg & (1ull << b)) p.reg_bits[k++] = b;
         passes.push_back(p);
@@ -272,6 +285,13 @@ __global__ __launch_bounds__(kBlock) void k_fused_adj(cx<R> *__restrict__ lam, c
                 } else {
                     adj_plain_dense<R, NR, false>(cur, nullptr, lane, a, b);
                 }
+            } else if (diag_fast(cur)) {
+                if (slot >= 0) {
+                    const CFGen *gp = cgens + slot;
+                    acc[slot][threadIdx.x] += adj_diag_step<R, NR, true>(cur, gp->g[0].re, gp->g[1].re, base, a, b);
+                } else {
+                    adj_diag_step<R, NR, false>(cur, 0.0, 0.0, base, a, b);
+                }
             } else {
                 if (slot >= 0) {
                     const FGen g = load_gen(cgens + slot);
@@ -329,8 +349,9 @@ FOp pack(const HOp &h, const uint64_t (&reg_off)[kMaxNR], int nr) {
     FOp f = h.f;
     bool real = h.kind != kDiag;
     for (int q = 0; q < 4 && real; ++q) real = f.m[q].im == 0.0;
+    const bool dsym = h.kind == kDiag && f.m[1].re == f.m[0].re && f.m[1].im == -f.m[0].im;
     f.meta = h.kind | uint32_t(h.loc) << 4 | uint32_t(h.slot + 1) << 8 | h.gkind << 16 | h.dneg << 20 |
-             uint32_t(real) << 21;
+             uint32_t(real) << 21 | uint32_t(dsym) << 22;
     uint32_t rok = 0, rpar = 0;
     for (int k = 0; k < kMaxNR; ++k) {
         // registers beyond nr do not exist; their bits read as "controls hold"
@@ -397,6 +418,9 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
             f.m[0] = to_cx<double>(k.d_even);
             f.m[1] = to_cx<double>(k.d_odd);
             h.dneg = is_neg_diag(k) ? 1 : 0;
+            // register bit of the parity mask (diagonal fast paths)
+            h.loc = h.pm_reg == 0 ? kDiagNoReg
+                  : __builtin_popcountll(h.pm_reg) == 1 ? reg_loc(__builtin_ctzll(h.pm_reg)) : 15;
             if (k.cmask == 0 && __builtin_popcountll(k.pmask) == 1 && (k.pmask & tilemask))
                 single_bit = __builtin_ctzll(k.pmask);
         } else {

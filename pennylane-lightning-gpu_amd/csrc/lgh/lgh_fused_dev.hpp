@@ -30,7 +30,7 @@ constexpr int kRBAdjoint = 3;  // adjoint passes (lambda and one observable stat
 // register k's offset), so each register's test is a compile-time bit of a
 // scalar.  The remaining bits are tested once per lane on the tile base.
 struct FOp {
-    uint32_t meta; // kind | loc << 4 | (slot + 1) << 8 | gkind << 16 | dneg << 20 | mreal << 21
+    uint32_t meta; // kind | loc << 4 | (slot + 1) << 8 | gkind << 16 | dneg << 20 | mreal << 21 | dsym << 22
     uint32_t rbits; // rok | rpar << 16
     uint64_t cm_rest, pm_rest;
     uint64_t pad;
@@ -48,6 +48,8 @@ __device__ __forceinline__ int f_slot(const FOp &o) { return int((o.meta >> 8) &
 __device__ __forceinline__ uint32_t f_gkind(const FOp &o) { return (o.meta >> 16) & 15u; }
 __device__ __forceinline__ bool f_dneg(const FOp &o) { return (o.meta >> 20) & 1u; }
 __device__ __forceinline__ bool f_real(const FOp &o) { return (o.meta >> 21) & 1u; }
+// diagonal with d_odd = conj(d_even) (RZ, CRZ, IsingZZ, MultiRZ)
+__device__ __forceinline__ bool f_dsym(const FOp &o) { return (o.meta >> 22) & 1u; }
 
 struct PassDesc {
     int32_t nops;
@@ -166,14 +168,68 @@ __device__ __forceinline__ void lane_masked(const FOp &op, const M2<R> &m, int l
     }
 }
 
+// x with its sign bit xor-ed by m (0 or 0x80000000)
+__device__ __forceinline__ float flip(float x, uint32_t m) { return __uint_as_float(__float_as_uint(x) ^ m); }
+__device__ __forceinline__ double flip(double x, uint32_t m) {
+    return __hiloint2double(int(uint32_t(__double2hiint(x)) ^ m), __double2loint(x));
+}
+constexpr uint32_t kSignBit = 0x80000000u;
+// location code of a diagonal gate whose parity mask has no register bit
+constexpr int kDiagNoReg = 7;
+
+// phase (c, s) where the parity of the mask bits is odd, (c, -s) where even;
+// the register part of the parity is register bit B (B < 0: none), so each
+// register's sign is known at compile time
+template <class R> __device__ __forceinline__ cx<R> phase_cs(cx<R> x, R c, R s) {
+    return cx<R>{c * x.re - s * x.im, c * x.im + s * x.re};
+}
+template <class R, int NR, int B>
+__device__ __forceinline__ void diag_sym(const FOp &op, bool lpar, cx<R> (&v)[NR]) {
+    const R c = R(op.m[1].re), s = R(op.m[1].im);
+    const R s0 = flip(s, lpar ? 0u : kSignBit); // registers with even register parity
+    const R s1 = flip(s, lpar ? kSignBit : 0u);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) v[k] = phase_cs(v[k], c, (B >= 0 && ((k >> B) & 1)) ? s1 : s0);
+}
+
 template <class R, int NR>
 __device__ __forceinline__ void diag_gate(const FOp &op, uint64_t base, bool lane_ok, cx<R> (&v)[NR]) {
     if (f_dneg(op)) {
+        // -1 where the controls hold (CZ, Z): flip the sign bits
+        const uint32_t ml = lane_ok ? kSignBit : 0u;
 #pragma unroll
-        for (int k = 0; k < NR; ++k) v[k] = sel(reg_ok(op, k) && lane_ok, neg(v[k]), v[k]);
+        for (int k = 0; k < NR; ++k) {
+            const uint32_t m = reg_ok(op, k) ? ml : 0u;
+            v[k] = cx<R>{flip(v[k].re, m), flip(v[k].im, m)};
+        }
         return;
     }
     const bool lpar = __popcll(base & op.pm_rest) & 1;
+    if (f_dsym(op) && f_plain(op)) {
+        switch (f_loc(op)) {
+        case 0: diag_sym<R, NR, 0>(op, lpar, v); return;
+        case 1: diag_sym<R, NR, 1>(op, lpar, v); return;
+        case 2: diag_sym<R, NR, 2>(op, lpar, v); return;
+        case 3:
+            if constexpr (NR > 8) {
+                diag_sym<R, NR, 3>(op, lpar, v);
+                return;
+            }
+            break;
+        case kDiagNoReg: diag_sym<R, NR, -1>(op, lpar, v); return;
+        default: break;
+        }
+        // parity over several register bits
+        const R c = R(op.m[1].re), s = R(op.m[1].im);
+        const uint32_t ml = lpar ? 0u : kSignBit;
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const R sk = flip(s, reg_par(op, k) ? (ml ^ kSignBit) : ml);
+            const cx<R> x = v[k];
+            v[k] = cx<R>{c * x.re - sk * x.im, c * x.im + sk * x.re};
+        }
+        return;
+    }
     cx<R> pe = lo<R>(lpar ? op.m[1] : op.m[0]); // phase when the register part is even
     cx<R> po = lo<R>(lpar ? op.m[0] : op.m[1]);
     if (!lane_ok) pe = po = cx<R>{R(1), R(0)};
@@ -383,6 +439,50 @@ __device__ __forceinline__ double adj_plain_dense(const FOp &op, const FGen *g, 
     case 3: return real ? adj_lane_plain<R, NR, 3, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 3, false, Gen>(m, gk, g, lane, a, b);
     case 4: return real ? adj_lane_plain<R, NR, 4, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 4, false, Gen>(m, gk, g, lane, a, b);
     default: return real ? adj_lane_plain<R, NR, 5, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 5, false, Gen>(m, gk, g, lane, a, b);
+    }
+}
+
+// One reverse step of an uncontrolled RZ-type diagonal gate on (lambda, bra):
+// the overlap with a diagonal generator (eigenvalue g_even / g_odd by the
+// parity of the mask bits) and the phase on both states, the lane parity
+// computed once.
+template <class R, int NR, int B, bool Gen>
+__device__ __forceinline__ double adj_diag_sym(const FOp &op, double ge, double go, uint64_t base, cx<R> (&a)[NR],
+                                               cx<R> (&b)[NR]) {
+    const bool lpar = __popcll(base & op.pm_rest) & 1;
+    const R c = R(op.m[1].re), s = R(op.m[1].im);
+    const R s0 = flip(s, lpar ? 0u : kSignBit), s1 = flip(s, lpar ? kSignBit : 0u);
+    const double g0 = lpar ? go : ge; // registers with even register parity
+    const double g1 = lpar ? ge : go;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        const bool odd = B >= 0 && ((k >> B) & 1);
+        if constexpr (Gen) acc += (odd ? g1 : g0) * im_cdot2(b[k], a[k]);
+        a[k] = phase_cs(a[k], c, odd ? s1 : s0);
+        b[k] = phase_cs(b[k], c, odd ? s1 : s0);
+    }
+    return acc;
+}
+
+// true when adj_diag_step handles the gate
+__device__ __forceinline__ bool diag_fast(const FOp &op) {
+    const int loc = f_loc(op);
+    return f_kind(op) == kDiag && f_dsym(op) && f_plain(op) && (loc < 4 || loc == kDiagNoReg) &&
+           (f_slot(op) < 0 || f_gkind(op) == kGenDiag);
+}
+// ge, go: generator eigenvalues for even / odd parity (unused without Gen)
+template <class R, int NR, bool Gen>
+__device__ __forceinline__ double adj_diag_step(const FOp &op, double ge, double go, uint64_t base, cx<R> (&a)[NR],
+                                                cx<R> (&b)[NR]) {
+    switch (f_loc(op)) {
+    case 0: return adj_diag_sym<R, NR, 0, Gen>(op, ge, go, base, a, b);
+    case 1: return adj_diag_sym<R, NR, 1, Gen>(op, ge, go, base, a, b);
+    case 2: return adj_diag_sym<R, NR, 2, Gen>(op, ge, go, base, a, b);
+    case 3:
+        if constexpr (NR > 8) return adj_diag_sym<R, NR, 3, Gen>(op, ge, go, base, a, b);
+        return 0.0;
+    default: return adj_diag_sym<R, NR, -1, Gen>(op, ge, go, base, a, b);
     }
 }
 
