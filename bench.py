@@ -54,6 +54,7 @@ def parse():
                     help="register bits of the fused forward / adjoint passes (library default 4 3)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-measure", action="store_true")
     ap.add_argument("--serve-arrivals", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-streams", type=int, default=4)
@@ -156,6 +157,56 @@ def bench_batched(N, args):
                              "max_abs_diff_vs_batched": diff},
         "speedup_vs_per_circuit": round(serial_full / float(np.median(st)), 1),
     }
+
+
+def bench_measurements(N, args):
+    """Measurement workloads, complex128: the state of 8 x [RX, RY, RZ on every
+    wire; CNOT ladder] from |0..0>.  (1) <H> of 256 random Pauli words (1-4
+    factors) on 24 qubits; (2) <H> of the open-chain TFIM (J = 1, h = 0.7) as
+    a resident CSR on 20 qubits; (3) 10^6 seeded samples on 24 qubits.  Times
+    are HIP events on the state's stream, median of repeats."""
+    import numpy as np
+
+    from pennylane_lightning_gpu.workloads import ladder, random_pauli_hamiltonian, tfim_csr
+
+    out = {}
+    n = 24
+    S = 16 << n
+    psi = N.State(n).reset()
+    psi.apply(ladder(n, 8, seed=args.seed))
+    timer = StreamTimer(psi.stream())
+    words, wires, coeffs = random_pauli_hamiltonian(n, 256, seed=args.seed)
+    psi.expval_pauli_words(words, wires, coeffs)  # warm
+    ms = sorted(timer.time(lambda: psi.expval_pauli_words(words, wires, coeffs)) for _ in range(5))[2]
+    # one pass over the state per group of at most 8 words sharing an X/Y mask
+    masks = {}
+    for wd, ws in zip(words, wires):
+        m = sum(1 << (n - 1 - w) for p, w in zip(wd, ws) if p in "XY")
+        masks[m] = masks.get(m, 0) + 1
+    passes = sum((c + 7) // 8 for c in masks.values())
+    out["pauli_words"] = {"qubits": n, "words": len(words), "ms": round(ms, 3), "passes": passes,
+                          "bytes": passes * S, "GBps": round(passes * S / (ms * 1e-3) / 1e9, 1),
+                          "words_per_s": round(len(words) / (ms * 1e-3), 1)}
+    shots = 10 ** 6
+    psi.sample(shots, args.seed)  # warm
+    ms = sorted(timer.time(lambda: psi.sample(shots, args.seed)) for _ in range(3))[1]
+    sbytes = S + 2 * 8 * (1 << n) + 8 * shots
+    out["sampling"] = {"qubits": n, "shots": shots, "ms": round(ms, 3), "bytes": sbytes,
+                       "GBps": round(sbytes / (ms * 1e-3) / 1e9, 1)}
+    del psi
+    n = 20
+    st = N.State(n).reset()
+    st.apply(ladder(n, 8, seed=args.seed))
+    indptr, indices, data = tfim_csr(n)
+    csr = N.CSR(indptr, indices, data)
+    csr.expval(st)  # warm
+    timer = StreamTimer(st.stream())
+    ms = sorted(timer.time(lambda: csr.expval(st)) for _ in range(5))[2]
+    nnz, NN = len(data), 1 << n
+    cbytes = 24 * nnz + 8 * (NN + 1) + 16 * NN
+    out["csr_tfim"] = {"qubits": n, "nnz": nnz, "ms": round(ms, 3), "bytes": cbytes,
+                       "GBps": round(cbytes / (ms * 1e-3) / 1e9, 1)}
+    return out
 
 
 def bench_serving(N, args):
@@ -321,6 +372,9 @@ def main():
     serving = None
     if not args.no_serve:
         serving = bench_serving(N, args)
+    measurements = None
+    if not args.no_measure:
+        measurements = bench_measurements(N, args)
 
     # ---------------------------------------------------------------- 30q single-gate sweep
     sweep = None
@@ -391,6 +445,7 @@ def main():
         "kernels": kernel_stats,
         "batched": batched,
         "serving": serving,
+        "measurements": measurements,
         "gate_sweep": sweep,
         "jacobian_checksum": float(np.sum(jac)),
     }

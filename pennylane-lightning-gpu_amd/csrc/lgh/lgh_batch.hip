@@ -149,7 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_prep(const CircDev *__restrict
             bool real = (f.meta & 15u) != kDiag;
             for (int q = 0; q < 4; ++q) real = real && f.m[q].im == 0.0;
             const bool dsym = (f.meta & 15u) == kDiag && f.m[1].re == f.m[0].re && f.m[1].im == -f.m[0].im;
-            f.meta = (f.meta & ~(3u << 21)) | (uint32_t(real) << 21) | (uint32_t(dsym) << 22);
+            // (parametric gates are never exactly X: bit 23 stays as compiled, clear)
+            f.meta = (f.meta & ~(7u << 21)) | (uint32_t(real) << 21) | (uint32_t(dsym) << 22);
         }
         cd.fops[i] = f;
     }
@@ -280,6 +281,8 @@ __device__ __forceinline__ void rev_pass(const CTPass *tp, const CFOp *fops, con
     }
 }
 
+// (two wavefronts per SIMD: forcing three spills registers inside the gate
+// loop and halves throughput)
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_batch(const CircDev *__restrict__ circs, int ncirc,
                                                   uint32_t *__restrict__ counter, int grad) {

@@ -244,10 +244,10 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
 // lane accumulates its overlap contributions in its own LDS column; the
 // reduction over lanes and wavefronts happens once at the end.
 template <class R, int RB>
-__global__ __launch_bounds__(kBlock) void k_fused_adj(cx<R> *__restrict__ lam, cx<R> *__restrict__ bra,
-                                                      uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
-                                                      const FOp *__restrict__ ops, const FGen *__restrict__ gens,
-                                                      double *__restrict__ partials) {
+__device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *__restrict__ bra, uint64_t ntiles,
+                                               uint64_t chunk, const PassDesc *__restrict__ pd,
+                                               const FOp *__restrict__ ops, const FGen *__restrict__ gens,
+                                               double *__restrict__ partials) {
     __shared__ double acc[kMaxSlots][kBlock];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const CFOp *cops = (const CFOp *)ops;
@@ -318,6 +318,23 @@ __global__ __launch_bounds__(kBlock) void k_fused_adj(cx<R> *__restrict__ lam, c
     }
 }
 
+// Three register bits fit three wavefronts per SIMD (168 VGPRs, 48 KiB LDS
+// per workgroup); four register bits need all registers of one.
+template <class R>
+__global__ __launch_bounds__(kBlock, 3) void k_fused_adj3(cx<R> *__restrict__ lam, cx<R> *__restrict__ bra,
+                                                       uint64_t ntiles, uint64_t chunk,
+                                                       const PassDesc *__restrict__ pd, const FOp *__restrict__ ops,
+                                                       const FGen *__restrict__ gens, double *__restrict__ partials) {
+    fused_adj_body<R, 3>(lam, bra, ntiles, chunk, pd, ops, gens, partials);
+}
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_fused_adj4(cx<R> *__restrict__ lam, cx<R> *__restrict__ bra,
+                                                    uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
+                                                    const FOp *__restrict__ ops, const FGen *__restrict__ gens,
+                                                    double *__restrict__ partials) {
+    fused_adj_body<R, 4>(lam, bra, ntiles, chunk, pd, ops, gens, partials);
+}
+
 // slot sums -> jac[col[s]] * factor[s]
 __global__ __launch_bounds__(kBlock) void k_slot_store(const double *__restrict__ partials, int nparts, int nslots,
                                                        const int64_t *__restrict__ cols,
@@ -350,8 +367,10 @@ FOp pack(const HOp &h, const uint64_t (&reg_off)[kMaxNR], int nr) {
     bool real = h.kind != kDiag;
     for (int q = 0; q < 4 && real; ++q) real = f.m[q].im == 0.0;
     const bool dsym = h.kind == kDiag && f.m[1].re == f.m[0].re && f.m[1].im == -f.m[0].im;
+    const bool isx = h.kind != kDiag && f.m[0].re == 0.0 && f.m[0].im == 0.0 && f.m[1].re == 1.0 &&
+                     f.m[1].im == 0.0 && f.m[2].re == 1.0 && f.m[2].im == 0.0 && f.m[3].re == 0.0 && f.m[3].im == 0.0;
     f.meta = h.kind | uint32_t(h.loc) << 4 | uint32_t(h.slot + 1) << 8 | h.gkind << 16 | h.dneg << 20 |
-             uint32_t(real) << 21 | uint32_t(dsym) << 22;
+             uint32_t(real) << 21 | uint32_t(dsym) << 22 | uint32_t(isx) << 23;
     uint32_t rok = 0, rpar = 0;
     for (int k = 0; k < kMaxNR; ++k) {
         // registers beyond nr do not exist; their bits read as "controls hold"
@@ -601,7 +620,7 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
         unsigned char *d = static_cast<unsigned char *>(ctx->upload(blob.data(), blob.size()));
         {
             ProfScope prof(kProfFusedAdj, ctx->stream);
-            auto kern = rb == 4 ? k_fused_adj<R, 4> : k_fused_adj<R, 3>;
+            auto kern = rb == 4 ? k_fused_adj4<R> : k_fused_adj3<R>;
             hipLaunchKernelGGL(kern, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream,
                                static_cast<cx<R> *>(lam->data), static_cast<cx<R> *>(bra->data), geo.ntiles,
                                geo.chunk, reinterpret_cast<const PassDesc *>(d), reinterpret_cast<const FOp *>(d + pb),

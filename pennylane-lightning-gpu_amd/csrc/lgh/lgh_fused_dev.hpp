@@ -19,7 +19,10 @@ namespace fz {
 constexpr int kMaxRB = 4;
 constexpr int kMaxNR = 1 << kMaxRB;
 constexpr int kLaneBits = 6;
-constexpr int kMaxSlots = 32;  // trainable gates per pass
+// trainable gates per pass: the adjoint kernels keep one double per slot and
+// lane in LDS (kMaxSlots x 256 x 8 B = 48 KiB per workgroup, three
+// workgroups per CU)
+constexpr int kMaxSlots = 24;
 constexpr int kRBForward = 4;  // forward passes (one state)
 constexpr int kRBAdjoint = 3;  // adjoint passes (lambda and one observable state)
 
@@ -50,6 +53,11 @@ __device__ __forceinline__ bool f_dneg(const FOp &o) { return (o.meta >> 20) & 1
 __device__ __forceinline__ bool f_real(const FOp &o) { return (o.meta >> 21) & 1u; }
 // diagonal with d_odd = conj(d_even) (RZ, CRZ, IsingZZ, MultiRZ)
 __device__ __forceinline__ bool f_dsym(const FOp &o) { return (o.meta >> 22) & 1u; }
+// dense 2x2 equal to [[0, 1], [1, 0]] (X, CNOT, Toffoli)
+__device__ __forceinline__ bool f_isx(const FOp &o) { return (o.meta >> 23) & 1u; }
+// every register satisfies the register-bit controls (the controls, if any,
+// are on lane or higher bits)
+__device__ __forceinline__ bool f_regs_ok(const FOp &o) { return (o.rbits & 0xffffu) == 0xffffu; }
 
 struct PassDesc {
     int32_t nops;
@@ -144,27 +152,39 @@ __device__ __forceinline__ void lane_plain(const M2<R> &m, int lane, cx<R> (&v)[
 }
 
 // --- controlled gates (controls on register bits and/or other bits) ------
-template <class R, int NR, int B>
+// AllReg: every register satisfies the register-bit controls; IsX: the
+// matrix is X, so the gate swaps the pair where the controls hold.
+template <class R, int NR, int B, bool AllReg, bool IsX>
 __device__ __forceinline__ void reg_masked(const FOp &op, const M2<R> &m, bool lane_ok, cx<R> (&v)[NR]) {
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
         if (k & (1 << B)) continue;
         const int k1 = k | (1 << B);
         cx<R> x = v[k], y = v[k1];
-        mat2<R, false>(m, x, y);
-        const bool ok = reg_ok(op, k) && lane_ok;
+        if constexpr (IsX) {
+            x = v[k1];
+            y = v[k];
+        } else {
+            mat2<R, false>(m, x, y);
+        }
+        const bool ok = (AllReg || reg_ok(op, k)) && lane_ok;
         v[k] = sel(ok, x, v[k]);
         v[k1] = sel(ok, y, v[k1]);
     }
 }
-template <class R, int NR, int L>
+template <class R, int NR, int L, bool AllReg, bool IsX>
 __device__ __forceinline__ void lane_masked(const FOp &op, const M2<R> &m, int lane, bool lane_ok, cx<R> (&v)[NR]) {
     const bool hi = (lane >> L) & 1;
     const cx<R> cs = hi ? m.m11 : m.m00, cp = hi ? m.m10 : m.m01;
 #pragma unroll
     for (int k = 0; k < NR; ++k) {
-        const cx<R> r = row2<R, false>(cs, cp, v[k], xor_lane_cx<L>(v[k], lane));
-        v[k] = sel(reg_ok(op, k) && lane_ok, r, v[k]);
+        const cx<R> p = xor_lane_cx<L>(v[k], lane);
+        cx<R> r;
+        if constexpr (IsX)
+            r = p;
+        else
+            r = row2<R, false>(cs, cp, v[k], p);
+        v[k] = sel((AllReg || reg_ok(op, k)) && lane_ok, r, v[k]);
     }
 }
 
@@ -197,10 +217,15 @@ __device__ __forceinline__ void diag_gate(const FOp &op, uint64_t base, bool lan
     if (f_dneg(op)) {
         // -1 where the controls hold (CZ, Z): flip the sign bits
         const uint32_t ml = lane_ok ? kSignBit : 0u;
+        if (f_regs_ok(op)) {
 #pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            const uint32_t m = reg_ok(op, k) ? ml : 0u;
-            v[k] = cx<R>{flip(v[k].re, m), flip(v[k].im, m)};
+            for (int k = 0; k < NR; ++k) v[k] = cx<R>{flip(v[k].re, ml), flip(v[k].im, ml)};
+        } else {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                const uint32_t m = reg_ok(op, k) ? ml : 0u;
+                v[k] = cx<R>{flip(v[k].re, m), flip(v[k].im, m)};
+            }
         }
         return;
     }
@@ -242,8 +267,19 @@ __device__ __forceinline__ void diag_gate(const FOp &op, uint64_t base, bool lan
 
 template <class R, int NR, int B> __device__ __forceinline__ void reg_gate(const FOp &op, bool lane_ok, cx<R> (&v)[NR]) {
     const M2<R> m = m2<R>(op);
-    if (!f_plain(op))
-        reg_masked<R, NR, B>(op, m, lane_ok, v);
+    if (!f_plain(op)) {
+        if (f_isx(op)) {
+            if (f_regs_ok(op))
+                reg_masked<R, NR, B, true, true>(op, m, lane_ok, v);
+            else
+                reg_masked<R, NR, B, false, true>(op, m, lane_ok, v);
+        } else {
+            if (f_regs_ok(op))
+                reg_masked<R, NR, B, true, false>(op, m, lane_ok, v);
+            else
+                reg_masked<R, NR, B, false, false>(op, m, lane_ok, v);
+        }
+    }
     else if (f_real(op))
         reg_plain<R, NR, B, true>(m, v);
     else
@@ -252,8 +288,19 @@ template <class R, int NR, int B> __device__ __forceinline__ void reg_gate(const
 template <class R, int NR, int L>
 __device__ __forceinline__ void lane_gate(const FOp &op, int lane, bool lane_ok, cx<R> (&v)[NR]) {
     const M2<R> m = m2<R>(op);
-    if (!f_plain(op))
-        lane_masked<R, NR, L>(op, m, lane, lane_ok, v);
+    if (!f_plain(op)) {
+        if (f_isx(op)) {
+            if (f_regs_ok(op))
+                lane_masked<R, NR, L, true, true>(op, m, lane, lane_ok, v);
+            else
+                lane_masked<R, NR, L, false, true>(op, m, lane, lane_ok, v);
+        } else {
+            if (f_regs_ok(op))
+                lane_masked<R, NR, L, true, false>(op, m, lane, lane_ok, v);
+            else
+                lane_masked<R, NR, L, false, false>(op, m, lane, lane_ok, v);
+        }
+    }
     else if (f_real(op))
         lane_plain<R, NR, L, true>(m, lane, v);
     else

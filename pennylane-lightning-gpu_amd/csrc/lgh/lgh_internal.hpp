@@ -110,22 +110,25 @@ template <class R> __device__ __forceinline__ cx<R> shfl_xor_cx(cx<R> a, int mas
     return cx<R>{__shfl_xor(a.re, mask, 64), __shfl_xor(a.im, mask, 64)};
 }
 
-// Value of lane (lane ^ 2^L) without going through LDS: DPP quad
-// permutations for L = 0, 1, two DPP row shifts for L = 2, 3 (row_shl:N reads
-// lane + N, row_shr:N reads lane - N inside a 16-lane row), and the gfx950
-// permlane swaps for L = 4, 5 (odd 16- or 32-lane rows of the first operand
-// trade places with even rows of the second).
+// Value of lane (lane ^ 2^L) without going through LDS: DPP moves inside
+// 16-lane rows for L <= 3 -- quad_perm [1,0,3,2] (xor 1), quad_perm
+// [2,3,0,1] (xor 2), row_half_mirror (xor 7 in each 8 lanes) followed by
+// quad_perm [3,2,1,0] (xor 3), together xor 4, and row_ror:8 (xor 8) -- and
+// the gfx950 permlane swaps for L = 4, 5 (odd 16- or 32-lane rows of the first
+// operand trade places with even rows of the second).  Every lane reads a
+// lane of its own row, so bound_ctrl (0 for out-of-row reads) never applies;
+// it lets the move skip initialising its destination.
 template <int L> __device__ __forceinline__ int xor_lane_b32(int v, int lane) {
     static_assert(L >= 0 && L < 6, "lane bit");
     if constexpr (L == 0) {
-        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false); // quad_perm [1,0,3,2]
+        return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, true); // quad_perm [1,0,3,2]
     } else if constexpr (L == 1) {
-        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false); // quad_perm [2,3,0,1]
-    } else if constexpr (L == 2 || L == 3) {
-        constexpr int n = 1 << L;
-        const int up = __builtin_amdgcn_update_dpp(0, v, 0x100 + n, 0xF, 0xF, false); // row_shl:n
-        const int dn = __builtin_amdgcn_update_dpp(0, v, 0x110 + n, 0xF, 0xF, false); // row_shr:n
-        return (lane & n) ? dn : up;
+        return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, true); // quad_perm [2,3,0,1]
+    } else if constexpr (L == 2) {
+        const int t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, true); // row_half_mirror
+        return __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, true);         // quad_perm [3,2,1,0]
+    } else if constexpr (L == 3) {
+        return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, true); // row_ror:8
     } else if constexpr (L == 4) {
         const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
         return int((lane & 16) ? p[0] : p[1]);
