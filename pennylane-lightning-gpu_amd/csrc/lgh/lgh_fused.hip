@@ -38,6 +38,11 @@ constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
 
 std::atomic<int> g_fusion{1};
+
+// Lane positions fixed to the lowest index bits: a load instruction then
+// covers runs of 128 consecutive bytes (8 complex doubles or 16 complex
+// floats), whole cache lines; the other lane positions are chosen per pass.
+int fixed_lanes(lgh_dtype dt) { return dt == LGH_C128 ? 3 : 4; }
 std::atomic<int> g_rb_fwd{kRBForward}, g_rb_adj{kRBAdjoint};
 
 struct Use {
@@ -74,7 +79,10 @@ bool fusable(const KOp &k, int n, int rb) {
 
 // Greedy list scheduling of `ops` (already compiled) into passes.
 std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot,
-                                  int rb) {
+                                  int rb, int fixed_lanes) {
+    LGH_REQUIRE(fixed_lanes >= 1 && fixed_lanes <= kLaneBits, "fused pass: bad number of fixed lane bits");
+    // index bits the scheduler chooses per pass: the free lane positions and the registers
+    const int nfree = kLaneBits - fixed_lanes + rb;
     const int N = static_cast<int>(ops.size());
     std::vector<Use> use(N);
     for (int i = 0; i < N; ++i) use[i] = uses_of(ops[i]);
@@ -113,84 +121,130 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
         }
     }
     std::vector<Pass> passes;
-    std::vector<char> done(N, 0);
-    std::vector<int> ready;
-    for (int i = 0; i < N; ++i)
-        if (indeg[i] == 0) ready.push_back(i);
-    int remaining = N;
-    auto release = [&](int i) {
-        done[i] = 1;
-        --remaining;
-        for (int s : succ[i])
-            if (--indeg[s] == 0) ready.push_back(s);
+    // list-scheduler state; copied to try out a register bit before taking it
+    struct State {
+        std::vector<int> indeg, ready;
+        int slots = 0;
     };
-    const uint64_t lane_mask = (1ull << kLaneBits) - 1;
+    State st;
+    st.indeg = indeg;
+    for (int i = 0; i < N; ++i)
+        if (indeg[i] == 0) st.ready.push_back(i);
+    int remaining = N;
+    const uint64_t lane_mask = (1ull << fixed_lanes) - 1;
+    std::vector<char> fus(N), dense(N), train(N);
+    for (int i = 0; i < N; ++i) {
+        fus[i] = fusable(ops[i], n, rb);
+        dense[i] = ops[i].kind == KOp::DENSE;
+        train[i] = !trainable_slot.empty() && trainable_slot[i] >= 0;
+    }
+    auto release = [&](State &s, int i) {
+        for (int t : succ[i])
+            if (--s.indeg[t] == 0) s.ready.push_back(t);
+    };
+    // Absorbs every ready gate whose targets are lane or register bits (and
+    // the gates this makes ready); returns the number of dense gates taken.
+    auto absorb = [&](State &s, uint64_t reg, std::vector<int> *taken) {
+        int nd = 0;
+        for (bool progress = true; progress;) {
+            progress = false;
+            for (size_t r = 0; r < s.ready.size();) {
+                const int i = s.ready[r];
+                if (!fus[i] || (use[i].nondiag & ~lane_mask & ~reg) || (train[i] && s.slots >= kMaxSlots)) {
+                    ++r;
+                    continue;
+                }
+                s.slots += train[i];
+                nd += dense[i];
+                if (taken) taken->push_back(i);
+                s.ready[r] = s.ready.back();
+                s.ready.pop_back();
+                release(s, i);
+                progress = true;
+            }
+        }
+        return nd;
+    };
+    // targets (outside the lane bits and `reg`) of ready dense gates
+    auto blocked_targets = [&](const State &s, uint64_t reg) {
+        uint64_t t = 0;
+        for (int i : s.ready)
+            if (fus[i] && dense[i] && !(train[i] && s.slots >= kMaxSlots)) t |= use[i].nondiag & ~lane_mask & ~reg;
+        return t;
+    };
     while (remaining > 0) {
-        std::sort(ready.begin(), ready.end());
         // a ready gate that cannot be fused runs on its own
-        if (!fusable(ops[ready.front()], n, rb)) {
-            const int i = ready.front();
-            ready.erase(ready.begin());
+        int solo = -1;
+        for (int i : st.ready)
+            if (!fus[i] && (solo < 0 || i < solo)) solo = i;
+        if (solo >= 0) {
             Pass p;
             p.standalone = true;
-            p.ops.push_back(i);
+            p.ops.push_back(solo);
             passes.push_back(p);
-            release(i);
+            st.ready.erase(std::find(st.ready.begin(), st.ready.end(), solo));
+            release(st, solo);
+            --remaining;
             continue;
         }
-        // Register bits are chosen while the pass grows: absorb every ready
-        // gate whose targets are lane or register bits; when none is left
-        // and a register bit is free, add the target of the earliest ready
-        // dense gate that is blocked, and absorb again.  A chain of gates
-        // (each one's target the next one's control) then fills a pass
-        // instead of starting a new one per gate.
+        // The free tile bits (free lane positions and registers) are chosen
+        // while the pass grows: absorb every ready gate whose targets are
+        // tile bits; while a tile bit is free, try each target of a blocked
+        // dense gate as that bit and keep the one that lets the most dense
+        // gates in (ties: the target of the earliest gate), then absorb
+        // again.  A chain of gates (each one's target the next one's
+        // control) then fills a pass instead of starting a new one per gate.
         Pass p;
         uint64_t reg = 0;
         int nreg = 0;
-        int slots = 0;
-        auto trainable = [&](int i) { return !trainable_slot.empty() && trainable_slot[i] >= 0; };
+        st.slots = 0;
         for (;;) {
-            bool progress = true;
-            while (progress) {
-                progress = false;
-                std::sort(ready.begin(), ready.end());
-                for (size_t r = 0; r < ready.size(); ++r) {
-                    const int i = ready[r];
-                    if (!fusable(ops[i], n, rb)) continue;
-                    const uint64_t t = use[i].nondiag & ~lane_mask;
-                    if (t & ~reg) continue;
-                    if (trainable(i) && slots >= kMaxSlots) continue;
-                    slots += trainable(i);
-                    p.ops.push_back(i);
-                    ready.erase(ready.begin() + r);
-                    release(i);
-                    progress = true;
-                    break;
+            absorb(st, reg, &p.ops);
+            if (nreg >= nfree) break;
+            const uint64_t cand = blocked_targets(st, reg);
+            if (!cand) break;
+            uint64_t best = 0;
+            long best_score = -1;
+            for (uint64_t c = cand; c; c &= c - 1) {
+                const uint64_t bit = c & (~c + 1);
+                State t = st;
+                long score = absorb(t, reg | bit, nullptr);
+                int first = N;
+                for (int i : st.ready)
+                    if (dense[i] && (use[i].nondiag & bit)) first = std::min(first, i);
+                score = score * (N + 1) + (N - first);
+                if (score > best_score) {
+                    best_score = score;
+                    best = bit;
                 }
             }
-            if (nreg >= rb) break;
-            uint64_t grow = 0;
-            for (int i : ready) { // sorted by the absorption loop
-                if (!fusable(ops[i], n, rb) || ops[i].kind != KOp::DENSE) continue;
-                if (trainable(i) && slots >= kMaxSlots) continue;
-                const uint64_t t = use[i].nondiag & ~lane_mask & ~reg;
-                if (t) {
-                    grow = t;
-                    break;
-                }
-            }
-            if (!grow) break;
-            reg |= grow;
-            nreg += __builtin_popcountll(grow);
+            reg |= best;
+            ++nreg;
         }
-        // pad with the lowest unused bits so that every pass has rb register bits
-        for (int b = kLaneBits; b < n && nreg < rb; ++b)
+        remaining -= static_cast<int>(p.ops.size());
+        // gates only depend on earlier gates: index order is a valid order
+        std::sort(p.ops.begin(), p.ops.end());
+        // pad with the lowest unused bits so that every pass has all its tile bits
+        for (int b = fixed_lanes; b < n && nreg < nfree; ++b)
             if (!(reg & (1ull << b))) {
                 reg |= 1ull << b;
                 ++nreg;
             }
-        for (int b = 0, k = 0; b < n; ++b)
-            if (reg & (1ull << b)) p.reg_bits[k++] = b;
+        // The bits with the most dense gates become register bits (plain
+        // register math); the others go to the free lane positions (cross-lane
+        // exchanges).  Ties: the lower bit is a lane bit.
+        std::vector<int> chosen, weight(size_t(n), 0);
+        for (int b = 0; b < n; ++b)
+            if (reg & (1ull << b)) chosen.push_back(b);
+        for (int i : p.ops)
+            if (dense[i]) weight[size_t(ops[size_t(i)].tbits[0])]++;
+        std::stable_sort(chosen.begin(), chosen.end(), [&](int x, int y) { return weight[size_t(x)] < weight[size_t(y)]; });
+        std::vector<int> lanes(chosen.begin(), chosen.begin() + (kLaneBits - fixed_lanes));
+        std::vector<int> regs(chosen.begin() + (kLaneBits - fixed_lanes), chosen.end());
+        std::sort(lanes.begin(), lanes.end());
+        std::sort(regs.begin(), regs.end());
+        for (int q = 0; q < kLaneBits; ++q) p.lane_bits[q] = q < fixed_lanes ? q : lanes[size_t(q - fixed_lanes)];
+        for (int r = 0; r < rb; ++r) p.reg_bits[r] = regs[size_t(r)];
         passes.push_back(p);
     }
     return passes;
@@ -209,20 +263,23 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
                                                        const FOp *__restrict__ ops) {
     const int lane = threadIdx.x & 63;
     const CFOp *cops = (const CFOp *)ops;
-    const uint64_t wave = xcd_block() * kWavesPerBlock + (threadIdx.x >> 6);
+    const uint64_t wave = xcd_block() * kWavesPerBlock + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint64_t t0 = wave * chunk, t1 = min(t0 + chunk, ntiles);
     const int nops = pd->nops;
     constexpr int NR = 1 << RB;
     uint64_t roff[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) roff[k] = pd->reg_off[k];
-    int rs[RB];
+    int fs[kMaxFree];
 #pragma unroll
-    for (int r = 0; r < RB; ++r) rs[r] = pd->reg_sorted[r];
-    for (uint64_t t = t0; t < t1; ++t) {
-        const uint64_t base = tile_base<RB>(t, rs) | uint64_t(lane);
+    for (int r = 0; r < kMaxFree; ++r) fs[r] = pd->free_sorted[r];
+    const uint64_t tmask = pd->tile_mask, loff = lane_offset(lane, pd);
+    // first index of the wavefront's tile, stepped from tile to tile
+    uint64_t g = tile_base_free(t0, pd->nfix, fs, pd->nfree);
+    for (uint64_t t = t0; t < t1; ++t, g = next_tile(g, tmask)) {
+        const uint64_t base = g | loff;
         cx<R> v[NR];
-        cx<R> *p = sv + base; // register bits are above the lane bits: | equals +
+        cx<R> *p = sv + base; // the tile bits are disjoint: | equals +
 #pragma unroll
         for (int k = 0; k < NR; ++k) v[k] = ldnt(p + roff[k]);
         const CFOp *opp = cops;
@@ -249,7 +306,7 @@ __device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *_
                                                const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                double *__restrict__ partials) {
     __shared__ double acc[kMaxSlots][kBlock];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const CFOp *cops = (const CFOp *)ops;
     const CFGen *cgens = (const CFGen *)gens;
     const int nslots = pd->nslots;
@@ -261,13 +318,16 @@ __device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *_
     uint64_t roff[NR];
 #pragma unroll
     for (int k = 0; k < NR; ++k) roff[k] = pd->reg_off[k];
-    int rs[RB];
+    int fs[kMaxFree];
 #pragma unroll
-    for (int r = 0; r < RB; ++r) rs[r] = pd->reg_sorted[r];
-    for (uint64_t t = t0; t < t1; ++t) {
-        const uint64_t base = tile_base<RB>(t, rs) | uint64_t(lane);
+    for (int r = 0; r < kMaxFree; ++r) fs[r] = pd->free_sorted[r];
+    const uint64_t tmask = pd->tile_mask, loff = lane_offset(lane, pd);
+    // first index of the wavefront's tile, stepped from tile to tile
+    uint64_t g = tile_base_free(t0, pd->nfix, fs, pd->nfree);
+    for (uint64_t t = t0; t < t1; ++t, g = next_tile(g, tmask)) {
+        const uint64_t base = g | loff;
         cx<R> a[NR], b[NR];
-        cx<R> *pa = lam + base, *pb = bra + base; // register bits are above the lane bits
+        cx<R> *pa = lam + base, *pb = bra + base; // the tile bits are disjoint
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             a[k] = ldnt(pa + roff[k]);
@@ -390,20 +450,35 @@ bool is_neg_diag(const KOp &k) {
 
 Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse,
                           const std::vector<KGen> *gens, const std::vector<int> *slot_of, int rb, bool merge_forward) {
-    (void)n;
     LGH_REQUIRE(rb >= 1 && rb <= kMaxRB, "fused pass: bad register-bit count");
     const int nr = 1 << rb;
     Built b{};
     b.pd.rb = rb;
-    uint64_t regmask = 0;
+    uint64_t regmask = 0, lanemask = 0;
+    for (int q = 0; q < kLaneBits; ++q) {
+        LGH_REQUIRE(p.lane_bits[q] >= 0 && p.lane_bits[q] < n && !(lanemask >> p.lane_bits[q] & 1),
+                    "fused pass: bad lane bits");
+        lanemask |= 1ull << p.lane_bits[q];
+        b.pd.lane_bits[q] = p.lane_bits[q];
+    }
     for (int r = 0; r < rb; ++r) {
-        LGH_REQUIRE(p.reg_bits[r] >= kLaneBits, "fused pass: register bit below the lane bits");
+        LGH_REQUIRE(p.reg_bits[r] >= 0 && p.reg_bits[r] < n && !((lanemask | regmask) >> p.reg_bits[r] & 1),
+                    "fused pass: bad register bits");
         b.pd.reg_bits[r] = p.reg_bits[r];
         b.pd.reg_sorted[r] = p.reg_bits[r];
         regmask |= 1ull << p.reg_bits[r];
     }
     std::sort(b.pd.reg_sorted, b.pd.reg_sorted + rb);
-    const uint64_t tilemask = regmask | ((1ull << kLaneBits) - 1);
+    const uint64_t tilemask = regmask | lanemask;
+    // lane positions that carry the lowest index bits stay out of the tile id
+    int nfix = 0;
+    while (nfix < kLaneBits && p.lane_bits[nfix] == nfix) ++nfix;
+    b.pd.nfix = nfix;
+    b.pd.nfree = 0;
+    for (int bit = nfix; bit < n; ++bit)
+        if (tilemask >> bit & 1) b.pd.free_sorted[b.pd.nfree++] = bit;
+    LGH_REQUIRE(b.pd.nfree <= kMaxFree, "fused pass: too many free tile bits");
+    b.pd.tile_mask = tilemask;
     for (int k = 0; k < nr; ++k) {
         uint64_t o = 0;
         for (int r = 0; r < rb; ++r)
@@ -413,6 +488,11 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
     auto reg_loc = [&](int bit) {
         for (int r = 0; r < rb; ++r)
             if (p.reg_bits[r] == bit) return r;
+        return -1;
+    };
+    auto lane_loc = [&](int bit) {
+        for (int q = 0; q < kLaneBits; ++q)
+            if (p.lane_bits[q] == bit) return q;
         return -1;
     };
     std::vector<int> order = p.ops;
@@ -446,9 +526,9 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
             const int t = k.tbits[0];
             h.cm_reg = k.cmask & regmask;
             f.cm_rest = k.cmask & ~regmask;
-            if (t < kLaneBits) {
+            if (lane_loc(t) >= 0) {
                 h.kind = kLaneDense;
-                h.loc = t;
+                h.loc = lane_loc(t);
             } else {
                 h.kind = kRegDense;
                 h.loc = reg_loc(t);
@@ -499,9 +579,9 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
                                      add(mul(mn[2], mo[0]), mul(mn[3], mo[2])), add(mul(mn[2], mo[1]), mul(mn[3], mo[3]))};
             const int bit = single_bit;
             prev = HOp{};
-            if (bit < kLaneBits) {
+            if (lane_loc(bit) >= 0) {
                 prev.kind = kLaneDense;
-                prev.loc = bit;
+                prev.loc = lane_loc(bit);
             } else {
                 prev.kind = kRegDense;
                 prev.loc = reg_loc(bit);
@@ -569,7 +649,7 @@ namespace lgh {
 
 void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops) {
     const int rb = g_rb_fwd.load();
-    const std::vector<Pass> passes = schedule_passes(kops, sv->n, {}, rb);
+    const std::vector<Pass> passes = schedule_passes(kops, sv->n, {}, rb, fixed_lanes(sv->dtype));
     if (sv->dtype == LGH_C128)
         run_forward<double>(sv, kops, passes, rb);
     else
@@ -577,7 +657,7 @@ void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops) {
 }
 
 int count_passes(const std::vector<KOp> &kops, int n) {
-    return static_cast<int>(schedule_passes(kops, n, {}, g_rb_fwd.load()).size());
+    return static_cast<int>(schedule_passes(kops, n, {}, g_rb_fwd.load(), fixed_lanes(LGH_C128)).size());
 }
 
 template <class R>
@@ -589,7 +669,7 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
     StreamCtx *ctx = lam->ctx;
     const int n = lam->n;
     const int rb = g_rb_adj.load();
-    const std::vector<Pass> passes = schedule_passes(udag, n, slot_col, rb);
+    const std::vector<Pass> passes = schedule_passes(udag, n, slot_col, rb, fixed_lanes(lam->dtype));
     const Geometry geo = geometry(n, lam->device, rb);
     for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
         const Pass &p = *it;

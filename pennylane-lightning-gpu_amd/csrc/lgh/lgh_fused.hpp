@@ -10,10 +10,14 @@
 namespace lgh {
 
 // One fused pass: gates (indices into the compiled op list, in execution
-// order) whose dense targets are lane bits (0..5) or one of the register
-// bits.  A standalone pass holds one gate that the fused kernels cannot run.
+// order) whose dense targets are lane bits or register bits.  Lane position
+// p (bit p of the lane id) carries index bit lane_bits[p]; the lowest lane
+// positions always carry the lowest index bits, so that every load covers
+// whole runs of consecutive amplitudes.  A standalone pass holds one gate
+// that the fused kernels cannot run.
 struct Pass {
     bool standalone = false;
+    int lane_bits[6] = {0, 1, 2, 3, 4, 5};
     int reg_bits[4] = {0, 0, 0, 0}; // the first rb entries are used
     std::vector<int> ops;
 };
@@ -25,9 +29,13 @@ int fusion_enabled();
 // rb: register bits of the passes (tile = 2^(6 + rb) amplitudes per wavefront)
 bool fusable(const KOp &k, int n, int rb);
 // trainable_slot[i] >= 0 marks gates whose generator overlap is accumulated
-// (at most 32 per pass); empty for the forward direction.
+// (at most kMaxSlots per pass); empty for the forward direction.
+// fixed_lanes: the lane positions 0..fixed_lanes-1 carry index bits
+// 0..fixed_lanes-1 in every pass; the scheduler chooses the index bits of the
+// other lane positions per pass, like the register bits (6: all lane bits
+// fixed).
 std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot,
-                                  int rb);
+                                  int rb, int fixed_lanes = 6);
 int count_passes(const std::vector<KOp> &kops, int n);
 
 // Forward application of a compiled op list through fused passes.

@@ -11,7 +11,8 @@
 namespace lgh {
 namespace fz {
 
-// A wavefront's tile: the 64 lanes carry index bits 0..5, and each lane holds
+// A wavefront's tile: the 64 lanes carry six index bits (the lowest ones, or
+// the lowest three or four and others chosen per pass), and each lane holds
 // NR = 2^RB registers, one per combination of the pass's RB register bits.
 // RB is a compile-time parameter of the kernels (3 or 4): four register bits
 // fuse more gates per pass, three leave room for two resident wavefronts per
@@ -59,18 +60,33 @@ __device__ __forceinline__ bool f_isx(const FOp &o) { return (o.meta >> 23) & 1u
 // are on lane or higher bits)
 __device__ __forceinline__ bool f_regs_ok(const FOp &o) { return (o.rbits & 0xffffu) == 0xffffu; }
 
+// tile bits above the fixed lane bits: at most 5 lane bits and the register bits
+constexpr int kMaxFree = kLaneBits - 1 + kMaxRB;
+
 struct PassDesc {
     int32_t nops;
     int32_t nslots;
     int32_t rb;                  // register bits of the pass
-    int32_t pad;
+    int32_t nfix;                // lane positions 0..nfix-1 carry index bits 0..nfix-1
     int32_t reg_bits[kMaxRB];
     int32_t reg_sorted[kMaxRB];  // register bits in ascending order
     uint64_t reg_off[kMaxNR];    // index offset of each register
+    int32_t lane_bits[kLaneBits]; // index bit of each lane position
+    int32_t nfree;               // tile bits besides the fixed lane bits
+    int32_t pad;
+    int32_t free_sorted[kMaxFree]; // those bits in ascending order
+    int32_t pad2;
+    uint64_t tile_mask;          // all tile bits (lane and register bits)
 };
 
-// First index of tile t: t's bits go around the lane bits (0..5) and the
-// register bits, which are all >= 6.
+// first index of the tile after the one at g (the next index with zeros at
+// every tile bit)
+__host__ __device__ __forceinline__ uint64_t next_tile(uint64_t g, uint64_t tile_mask) {
+    return ((g | tile_mask) + 1) & ~tile_mask;
+}
+
+// First index of tile t when the lanes carry index bits 0..5: t's bits go
+// around the lane bits and the register bits, which are all >= 6.
 template <int RB> __device__ __forceinline__ uint64_t tile_base(uint64_t t, const int (&rs)[RB]) {
     uint64_t g = t << kLaneBits;
 #pragma unroll
@@ -79,6 +95,28 @@ template <int RB> __device__ __forceinline__ uint64_t tile_base(uint64_t t, cons
         g = ((g >> p) << (p + 1)) | (g & ((1ull << p) - 1ull));
     }
     return g;
+}
+
+// First index of tile t for any lane bits: t's bits go around the fixed lane
+// bits (0..nfix-1) and the nfree other tile bits fs[] (ascending).
+__host__ __device__ __forceinline__ uint64_t tile_base_free(uint64_t t, int nfix, const int (&fs)[kMaxFree], int nfree) {
+    uint64_t g = t << nfix;
+#pragma unroll
+    for (int r = 0; r < kMaxFree; ++r)
+        if (r < nfree) {
+            const int p = fs[r];
+            g = ((g >> p) << (p + 1)) | (g & ((1ull << p) - 1ull));
+        }
+    return g;
+}
+
+// index offset of a lane: bit p of the lane id goes to index bit lane_bits[p]
+__host__ __device__ __forceinline__ uint64_t lane_offset(int lane, const PassDesc *pd) {
+    uint64_t off = 0;
+#pragma unroll
+    for (int p = 0; p < kLaneBits; ++p)
+        if ((lane >> p) & 1) off |= 1ull << pd->lane_bits[p];
+    return off;
 }
 
 template <class R> __device__ __forceinline__ cx<R> lo(cx<double> z) { return cx<R>{R(z.re), R(z.im)}; }

@@ -623,6 +623,13 @@ void pauli_t(const lgh_sv *sv, int64_t nwords, const char *paulis, const int32_t
     }
     std::vector<double> vals(size_t(nwords), 0.0);
     const uint64_t len = sv->length();
+    // every pass reduces into its own result slots; one copy to the host at the end
+    const unsigned max_blocks = std::max(chunks_for(len, sv->device).blocks, chunks_for(len / 2, sv->device).blocks);
+    const size_t part_doubles = size_t(2 * kMaxWordsPerPass) * max_blocks;
+    char *ws_base = static_cast<char *>(ctx->workspace(sizeof(double) * (part_doubles + 2 * size_t(nwords)) + 256));
+    double *partials = reinterpret_cast<double *>(ws_base);
+    double *results = partials + part_doubles;
+    std::vector<int64_t> slot_word; // word of each result slot pair
     for (auto &kv : groups) {
         const uint64_t flip = kv.first;
         const int pivot = flip ? __builtin_ctzll(flip) : -1;
@@ -632,28 +639,31 @@ void pauli_t(const lgh_sv *sv, int64_t nwords, const char *paulis, const int32_t
         for (size_t b0 = 0; b0 < ids.size(); b0 += kMaxWordsPerPass) {
             WordSet ws{};
             ws.count = int(std::min<size_t>(kMaxWordsPerPass, ids.size() - b0));
-            for (int q = 0; q < ws.count; ++q) ws.zy[q] = words[size_t(ids[b0 + q])].zy;
-            RedBuf rb = red_buf(ctx, 2 * ws.count, ch.blocks);
-            hipLaunchKernelGGL(k_pauli<R>, dim3(ch.blocks), dim3(kBlock), 0, ctx->stream, cdata<R>(sv), count,
-                               ch.chunk, pivot, flip, ws, rb.partials);
-            LGH_LAUNCH_CHECK();
-            launch_reduce(ctx, rb.partials, ch.blocks, 2 * ws.count, rb.result);
-            double out[2 * kMaxWordsPerPass];
-            fetch(ctx, rb.result, 2 * ws.count, out);
             for (int q = 0; q < ws.count; ++q) {
-                const W &x = words[size_t(ids[b0 + q])];
-                // <P> = i^{nY} (re + i im): real part
-                const double re = out[2 * q], im = out[2 * q + 1];
-                double v;
-                switch (x.ny & 3) {
-                case 0: v = re; break;
-                case 1: v = -im; break;
-                case 2: v = -re; break;
-                default: v = im; break;
-                }
-                vals[size_t(ids[b0 + q])] = v;
+                ws.zy[q] = words[size_t(ids[b0 + q])].zy;
+                slot_word.push_back(ids[b0 + q]);
             }
+            hipLaunchKernelGGL(k_pauli<R>, dim3(ch.blocks), dim3(kBlock), 0, ctx->stream, cdata<R>(sv), count,
+                               ch.chunk, pivot, flip, ws, partials);
+            LGH_LAUNCH_CHECK();
+            launch_reduce(ctx, partials, ch.blocks, 2 * ws.count,
+                          results + 2 * (slot_word.size() - size_t(ws.count)));
         }
+    }
+    std::vector<double> out(2 * size_t(nwords));
+    if (nwords > 0) fetch(ctx, results, int(2 * nwords), out.data());
+    for (size_t q = 0; q < slot_word.size(); ++q) {
+        const W &x = words[size_t(slot_word[q])];
+        // <P> = i^{nY} (re + i im): real part
+        const double re = out[2 * q], im = out[2 * q + 1];
+        double v;
+        switch (x.ny & 3) {
+        case 0: v = re; break;
+        case 1: v = -im; break;
+        case 2: v = -re; break;
+        default: v = im; break;
+        }
+        vals[size_t(slot_word[q])] = v;
     }
     double tot = 0.0;
     for (int64_t w = 0; w < nwords; ++w) {
