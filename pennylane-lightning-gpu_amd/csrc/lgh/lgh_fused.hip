@@ -22,6 +22,7 @@
 // AdjointJacobianGPU::adjointJacobian (algorithms/AdjointDiffGPU.hpp:499-596);
 // the reference applies one cuStateVec kernel per gate and has no fusion.
 #include <algorithm>
+#include <complex>
 #include <atomic>
 #include <cstring>
 #include <map>
@@ -282,12 +283,18 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
         cx<R> *p = sv + base; // the tile bits are disjoint: | equals +
 #pragma unroll
         for (int k = 0; k < NR; ++k) v[k] = ldnt(p + roff[k]);
-        const CFOp *opp = cops;
-        FOp cur = load_op(opp);
-        for (int o = 0; o < nops; ++o) {
-            const FOp nxt = load_op(opp + (o + 1 < nops ? o + 1 : o)); // prefetch the next descriptor
+        FOp cur = load_op(cops);
+        for (int o = 0; o < nops;) {
+            if (f_kind(cur) == kDiagRun) {
+                diag_run<R, NR>(cops + o, cur, base, v);
+                o += 1 + run_ndata(cur) + run_nmem(cur);
+                cur = load_op(cops + (o < nops ? o : 0));
+                continue;
+            }
+            const FOp nxt = load_op(cops + (o + 1 < nops ? o + 1 : o)); // prefetch the next descriptor
             apply_fop<R, NR>(cur, lane, base, v);
             cur = nxt;
+            ++o;
         }
         // recompute the store addresses instead of keeping NR of them live
         asm volatile("" : "+v"(p));
@@ -333,10 +340,15 @@ __device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *_
             a[k] = ldnt(pa + roff[k]);
             b[k] = ldnt(pb + roff[k]);
         }
-        const CFOp *opp = cops;
-        FOp cur = load_op(opp);
-        for (int o = 0; o < nops; ++o) {
-            const FOp nxt = load_op(opp + (o + 1 < nops ? o + 1 : o));
+        FOp cur = load_op(cops);
+        for (int o = 0; o < nops;) {
+            if (f_kind(cur) == kDiagRun) {
+                diag_run_adj<R, NR, RB>(cops + o, cur, cgens, base, a, b, acc, threadIdx.x);
+                o += 1 + run_ndata(cur) + run_nmem(cur);
+                cur = load_op(cops + (o < nops ? o : 0));
+                continue;
+            }
+            const FOp nxt = load_op(cops + (o + 1 < nops ? o + 1 : o));
             const int slot = f_slot(cur);
             if (f_kind(cur) != kDiag && f_plain(cur)) {
                 if (slot >= 0) {
@@ -361,6 +373,7 @@ __device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *_
                 apply_fop<R, NR>(cur, lane, base, b);
             }
             cur = nxt;
+            ++o;
         }
         asm volatile("" : "+v"(pa), "+v"(pb));
 #pragma unroll
@@ -449,7 +462,8 @@ bool is_neg_diag(const KOp &k) {
 } // namespace
 
 Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse,
-                          const std::vector<KGen> *gens, const std::vector<int> *slot_of, int rb, bool merge_forward) {
+                          const std::vector<KGen> *gens, const std::vector<int> *slot_of, int rb, bool merge_forward,
+                          bool diag_runs) {
     LGH_REQUIRE(rb >= 1 && rb <= kMaxRB, "fused pass: bad register-bit count");
     const int nr = 1 << rb;
     Built b{};
@@ -594,7 +608,69 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
         hops.push_back(h);
         if (merge && single_bit >= 0) last_single[size_t(single_bit)] = int(hops.size()) - 1;
     }
-    for (const HOp &h : hops) b.ops.push_back(pack(h, b.pd.reg_off, nr));
+    for (size_t i = 0; i < hops.size();) {
+        // a diagonal member needs a diagonal generator when it is trainable
+        auto member = [&](const HOp &h) { return h.kind == kDiag && (h.slot < 0 || h.gkind == kGenDiag); };
+        size_t j = i;
+        while (diag_runs && j < hops.size() && member(hops[j])) ++j;
+        int deferred = 0;
+        for (size_t t = i; t < j; ++t) {
+            const HOp &h = hops[t];
+            const bool reg = (h.cm_reg | h.pm_reg) != 0, rest = (h.f.cm_rest | h.f.pm_rest) != 0;
+            deferred += !reg || !rest || h.dneg;
+        }
+        if (deferred < 2) {
+            b.ops.push_back(pack(hops[i], b.pd.reg_off, nr));
+            ++i;
+            continue;
+        }
+        // diagonal run over hops[i, j): header, P[k] data ops, members
+        std::vector<std::complex<double>> P(size_t(nr), 1.0);
+        uint32_t flags = 0;
+        std::vector<FOp> members;
+        for (size_t t = i; t < j; ++t) {
+            const HOp &h = hops[t];
+            const bool reg = (h.cm_reg | h.pm_reg) != 0, rest = (h.f.cm_rest | h.f.pm_rest) != 0;
+            uint32_t cls, ov = kOvF;
+            if (!reg) {
+                cls = kMemQ;
+            } else if (!rest) {
+                cls = kMemR;
+                flags |= kRunHasP;
+                const std::complex<double> de(h.f.m[0].re, h.f.m[0].im), dodd(h.f.m[1].re, h.f.m[1].im);
+                for (int k = 0; k < nr; ++k) {
+                    if ((b.pd.reg_off[k] & h.cm_reg) != h.cm_reg) continue;
+                    P[size_t(k)] *= (__builtin_popcountll(b.pd.reg_off[k] & h.pm_reg) & 1) ? dodd : de;
+                }
+            } else {
+                cls = h.dneg ? kMemS : kMemX;
+                if (h.dneg) flags |= kRunHasS;
+            }
+            if (h.slot >= 0 && h.cm_reg == 0) {
+                if (h.pm_reg == 0) ov = kOvT;
+                else if (__builtin_popcountll(h.pm_reg) == 1 && h.loc < rb) ov = kOvB;
+                if (ov != kOvF) flags |= kRunNeedsTE;
+            }
+            FOp f = pack(h, b.pd.reg_off, nr);
+            f.meta |= cls << 24 | ov << 26;
+            members.push_back(f);
+        }
+        const int ndata = (flags & kRunHasP) ? (nr + 3) / 4 : 0;
+        FOp hdr{};
+        hdr.meta = kDiagRun;
+        hdr.pad = uint64_t(members.size()) | uint64_t(ndata) << 16 | uint64_t(flags) << 24;
+        b.ops.push_back(hdr);
+        for (int d = 0; d < ndata; ++d) {
+            FOp data{};
+            for (int q = 0; q < 4; ++q) {
+                const int k = 4 * d + q;
+                if (k < nr) data.m[q] = cx<double>{P[size_t(k)].real(), P[size_t(k)].imag()};
+            }
+            b.ops.push_back(data);
+        }
+        b.ops.insert(b.ops.end(), members.begin(), members.end());
+        i = j;
+    }
     b.pd.nops = static_cast<int32_t>(b.ops.size());
     b.pd.nslots = nslots;
     return b;
@@ -628,7 +704,7 @@ void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pas
             launch_kop(sv, kops[size_t(p.ops[0])]);
             continue;
         }
-        Built b = build_pass(p, kops, sv->n, false, nullptr, nullptr, rb);
+        Built b = build_pass(p, kops, sv->n, false, nullptr, nullptr, rb, true, true);
         const size_t pb = (sizeof(PassDesc) + 255) & ~size_t(255);
         std::vector<unsigned char> blob(pb + b.ops.size() * sizeof(FOp));
         std::memcpy(blob.data(), &b.pd, sizeof(PassDesc));
@@ -679,7 +755,7 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
                        factor[size_t(i)]);
             continue;
         }
-        Built b = build_pass(p, udag, n, true, &gens, &slot_col, rb);
+        Built b = build_pass(p, udag, n, true, &gens, &slot_col, rb, true, true);
         std::vector<int64_t> cols;
         std::vector<double> facs;
         for (auto oi = p.ops.rbegin(); oi != p.ops.rend(); ++oi)

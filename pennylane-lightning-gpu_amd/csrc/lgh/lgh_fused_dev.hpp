@@ -44,7 +44,7 @@ struct FOp {
 struct FGen {
     cx<double> g[4]; // dense generator, or (g_even, g_odd) in g[0].re, g[1].re
 };
-enum : uint32_t { kLaneDense = 0, kRegDense = 1, kDiag = 2 };
+enum : uint32_t { kLaneDense = 0, kRegDense = 1, kDiag = 2, kDiagRun = 3 };
 enum : uint32_t { kGenDense = 0, kGenDiag = 1, kGenX = 2, kGenY = 3 };
 __device__ __forceinline__ uint32_t f_kind(const FOp &o) { return o.meta & 15u; }
 __device__ __forceinline__ int f_loc(const FOp &o) { return int((o.meta >> 4) & 15u); }
@@ -59,6 +59,34 @@ __device__ __forceinline__ bool f_isx(const FOp &o) { return (o.meta >> 23) & 1u
 // every register satisfies the register-bit controls (the controls, if any,
 // are on lane or higher bits)
 __device__ __forceinline__ bool f_regs_ok(const FOp &o) { return (o.rbits & 0xffffu) == 0xffffu; }
+
+// Diagonal runs.  Diagonal gates commute, so a run of consecutive diagonal
+// gates of a pass is applied as one phase per amplitude: the product of the
+// members whose bits are all outside the registers is a per-lane factor q,
+// the product of the members whose bits are all register bits is a constant
+// P[k] per register (computed on the host), members that negate where their
+// controls hold (CZ) with bits on both sides flip a per-register sign mask,
+// and the other members are applied as they come.  At the end of the run
+// every register is multiplied once by q * P[k] (with its sign).  In the
+// adjoint sweep both states get the same diagonal unitaries, which leave
+// conj(b_x) a_x unchanged, so every member's generator overlap can be formed
+// from w_k = Im(conj(b_k) a_k) at any point of the run: T = sum_k w_k and
+// E_B = sum of w_k over the registers with register bit B clear give the
+// overlaps of generators that do not depend on the registers or depend on
+// one register bit.
+//
+// The run is stored as a header op (kind kDiagRun; pad = members |
+// data ops << 16 | flags << 24), then data ops holding P[k] in their m[]
+// (four registers per op, only with kRunHasP), then the members, whose meta
+// carries their class (bits 24-25) and overlap form (bits 26-27).
+enum : uint32_t { kMemQ = 0, kMemR = 1, kMemS = 2, kMemX = 3 };
+enum : uint32_t { kOvT = 0, kOvB = 1, kOvF = 2 };
+enum : uint32_t { kRunHasP = 1, kRunNeedsTE = 2, kRunHasS = 4 };
+__device__ __forceinline__ uint32_t f_mclass(const FOp &o) { return (o.meta >> 24) & 3u; }
+__device__ __forceinline__ uint32_t f_oclass(const FOp &o) { return (o.meta >> 26) & 3u; }
+__device__ __forceinline__ int run_nmem(const FOp &o) { return int(o.pad & 0xffffu); }
+__device__ __forceinline__ int run_ndata(const FOp &o) { return int((o.pad >> 16) & 0xffu); }
+__device__ __forceinline__ uint32_t run_flags(const FOp &o) { return uint32_t(o.pad >> 24) & 0xffu; }
 
 // tile bits above the fixed lane bits: at most 5 lane bits and the register bits
 constexpr int kMaxFree = kLaneBits - 1 + kMaxRB;
@@ -609,7 +637,7 @@ __device__ __forceinline__ FOp load_op(const CFOp *p) {
     f.rbits = p->rbits;
     f.cm_rest = p->cm_rest;
     f.pm_rest = p->pm_rest;
-    f.pad = 0;
+    f.pad = p->pad;
 #pragma unroll
     for (int q = 0; q < 4; ++q) f.m[q] = cx<double>{p->m[q].re, p->m[q].im};
     return f;
@@ -619,6 +647,141 @@ __device__ __forceinline__ FGen load_gen(const CFGen *p) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) g.g[q] = cx<double>{p->g[q].re, p->g[q].im};
     return g;
+}
+
+// --- diagonal runs ---------------------------------------------------------
+// per-lane factor of a member without register bits
+template <class R> __device__ __forceinline__ cx<R> run_q(const FOp &m, uint64_t base, cx<R> q) {
+    const bool lane_ok = (base & m.cm_rest) == m.cm_rest;
+    if (f_dneg(m)) {
+        const uint32_t fm = lane_ok ? kSignBit : 0u;
+        return cx<R>{flip(q.re, fm), flip(q.im, fm)};
+    }
+    const bool lpar = __popcll(base & m.pm_rest) & 1;
+    cx<R> d = lo<R>(lpar ? m.m[1] : m.m[0]);
+    if (!lane_ok) d = cx<R>{R(1), R(0)};
+    return cmul(q, d);
+}
+// sign mask update of a negating member with register and other bits
+__device__ __forceinline__ uint32_t run_s(const FOp &m, uint64_t base, uint32_t sm) {
+    return sm ^ (((base & m.cm_rest) == m.cm_rest) ? (m.rbits & 0xffffu) : 0u);
+}
+// factor of register K at the end of a run
+template <class R, int K>
+__device__ __forceinline__ cx<R> run_factor(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm) {
+    cx<R> f = q;
+    if (flags & kRunHasP) f = cmul(f, lo<R>(cx<double>{hp[1 + K / 4].m[K % 4].re, hp[1 + K / 4].m[K % 4].im}));
+    if (flags & kRunHasS) {
+        const uint32_t fm = (sm << (31 - K)) & kSignBit;
+        f = cx<R>{flip(f.re, fm), flip(f.im, fm)};
+    }
+    return f;
+}
+template <class R, int NR, int K = 0>
+__device__ __forceinline__ void run_finish(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR]) {
+    if constexpr (K < NR) {
+        a[K] = cmul(a[K], run_factor<R, K>(hp, flags, q, sm));
+        run_finish<R, NR, K + 1>(hp, flags, q, sm, a);
+    }
+}
+template <class R, int NR, int K = 0>
+__device__ __forceinline__ void run_finish2(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR],
+                                            cx<R> (&b)[NR]) {
+    if constexpr (K < NR) {
+        const cx<R> f = run_factor<R, K>(hp, flags, q, sm);
+        a[K] = cmul(a[K], f);
+        b[K] = cmul(b[K], f);
+        run_finish2<R, NR, K + 1>(hp, flags, q, sm, a, b);
+    }
+}
+
+// Forward: the run whose header hdr is at hp.
+template <class R, int NR>
+__device__ __forceinline__ void diag_run(const CFOp *hp, const FOp &hdr, uint64_t base, cx<R> (&v)[NR]) {
+    const int nmem = run_nmem(hdr);
+    const uint32_t flags = run_flags(hdr);
+    const CFOp *mp = hp + 1 + run_ndata(hdr);
+    cx<R> q{R(1), R(0)};
+    uint32_t sm = 0;
+    FOp m = load_op(mp);
+    for (int j = 0; j < nmem; ++j) {
+        const FOp mn = load_op(mp + (j + 1 < nmem ? j + 1 : j));
+        switch (f_mclass(m)) {
+        case kMemQ: q = run_q(m, base, q); break;
+        case kMemR: break;
+        case kMemS: sm = run_s(m, base, sm); break;
+        default: diag_gate<R, NR>(m, base, (base & m.cm_rest) == m.cm_rest, v); break;
+        }
+        m = mn;
+    }
+    run_finish<R, NR>(hp, flags, q, sm, v);
+}
+
+// Adjoint: the run applied to lambda (a) and the observable state (b); the
+// overlap of every trainable member is added to its slot of acc.
+template <class R, int NR, int RB>
+__device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, const CFGen *cgens, uint64_t base,
+                                             cx<R> (&a)[NR], cx<R> (&b)[NR], double (*acc)[256], int col) {
+    const int nmem = run_nmem(hdr);
+    const uint32_t flags = run_flags(hdr);
+    const CFOp *mp = hp + 1 + run_ndata(hdr);
+    double T = 0.0, E[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) E[r] = 0.0;
+    if (flags & kRunNeedsTE) {
+#pragma unroll
+        for (int k = 0; k < NR; ++k) {
+            const double w = im_cdot2(b[k], a[k]);
+            T += w;
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+                if (!((k >> r) & 1)) E[r] += w;
+        }
+    }
+    cx<R> q{R(1), R(0)};
+    uint32_t sm = 0;
+    FOp m = load_op(mp);
+    for (int j = 0; j < nmem; ++j) {
+        const FOp mn = load_op(mp + (j + 1 < nmem ? j + 1 : j));
+        const int slot = f_slot(m);
+        if (slot >= 0) {
+            const CFGen *gp = cgens + slot;
+            const uint32_t oc = f_oclass(m);
+            const bool lane_ok = (base & m.cm_rest) == m.cm_rest;
+            double val;
+            if (oc == kOvF) {
+                val = diag_dot<R, NR>(m, load_gen(gp), base, lane_ok, a, b);
+            } else {
+                const double ge = gp->g[0].re, go = gp->g[1].re;
+                const bool lpar = __popcll(base & m.pm_rest) & 1;
+                const double g0 = lpar ? go : ge, g1 = lpar ? ge : go;
+                if (oc == kOvT) {
+                    val = g0 * T;
+                } else {
+                    double e = E[0];
+#pragma unroll
+                    for (int r = 1; r < RB; ++r)
+                        if (f_loc(m) == r) e = E[r];
+                    val = g0 * e + g1 * (T - e);
+                }
+                val = lane_ok ? val : 0.0;
+            }
+            acc[slot][col] += val;
+        }
+        switch (f_mclass(m)) {
+        case kMemQ: q = run_q(m, base, q); break;
+        case kMemR: break;
+        case kMemS: sm = run_s(m, base, sm); break;
+        default: {
+            const bool lane_ok = (base & m.cm_rest) == m.cm_rest;
+            diag_gate<R, NR>(m, base, lane_ok, a);
+            diag_gate<R, NR>(m, base, lane_ok, b);
+            break;
+        }
+        }
+        m = mn;
+    }
+    run_finish2<R, NR>(hp, flags, q, sm, a, b);
 }
 
 // Host: device description of one pass (descriptor, gates, and for the
@@ -633,8 +796,10 @@ struct Built {
 // trainable gates (nullptr in the forward direction, which instead merges
 // consecutive single-qubit gates on one bit unless merge is false; without
 // merging, ops[j] is the j-th gate of the pass in execution order).
+// diag_runs: group consecutive diagonal gates into diagonal runs (the
+// single-state kernels; the batched kernels take plain gates).
 Built build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse, const std::vector<KGen> *gens,
-                 const std::vector<int> *slot_of, int rb, bool merge = true);
+                 const std::vector<int> *slot_of, int rb, bool merge = true, bool diag_runs = false);
 
 } // namespace fz
 } // namespace lgh

@@ -86,15 +86,22 @@ template <class R> cx<R> to_cx(std::complex<double> z) {
 // keeping it out of the caches leaves them to the data that is re-used.
 template <class R> using vec2_t = R __attribute__((ext_vector_type(2)));
 
+// State vectors live in global memory: the accesses name that address space,
+// so that they compile to global (not flat) instructions even where the
+// pointer went through an opaque register copy.  A flat access also counts
+// as an LDS / scalar-memory access, and every wait for a descriptor load
+// would then wait for the tile's stores as well.
+template <class R> using gvec2_t = __attribute__((address_space(1))) vec2_t<R>;
+
 template <class R> __device__ __forceinline__ cx<R> ldnt(const cx<R> *p) {
-    const vec2_t<R> v = __builtin_nontemporal_load(reinterpret_cast<const vec2_t<R> *>(p));
+    const vec2_t<R> v = __builtin_nontemporal_load((const gvec2_t<R> *)(p));
     return cx<R>{v.x, v.y};
 }
 template <class R> __device__ __forceinline__ void stnt(cx<R> *p, cx<R> a) {
     vec2_t<R> v;
     v.x = a.re;
     v.y = a.im;
-    __builtin_nontemporal_store(v, reinterpret_cast<vec2_t<R> *>(p));
+    __builtin_nontemporal_store(v, (gvec2_t<R> *)(p));
 }
 
 // XCD-aware workgroup order: the hardware hands consecutive workgroups to the

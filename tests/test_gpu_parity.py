@@ -397,6 +397,51 @@ def test_fused_adjoint_matches_oracle(n):
         N.set_fusion(True)
 
 
+def diagonal_heavy_circuit(n, rng, count):
+    """Mostly diagonal gates (long diagonal runs inside the fused passes,
+    with register, lane and other bits mixed) and a few RY that end runs."""
+    one = ["RZ", "PhaseShift", "S", "T", "PauliZ", "RY"]
+    two = ["CZ", "CRZ", "IsingZZ", "ControlledPhaseShift"]
+    ops = []
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.45:
+            g = one[rng.integers(len(one))]
+            w = [int(rng.integers(n))]
+        elif r < 0.9:
+            g = two[rng.integers(len(two))]
+            w = [int(x) for x in rng.choice(n, 2, replace=False)]
+        else:
+            g = "MultiRZ"
+            w = [int(x) for x in rng.choice(n, 3, replace=False)]
+        ops.append((g, w, list(rng.uniform(-3, 3, N.PARAM_COUNT.get(g, 0))), bool(rng.integers(2))))
+    return ops
+
+
+@pytest.mark.parametrize("bits", [(4, 3), (3, 3)])
+@pytest.mark.parametrize("n", [12, 13])
+def test_fused_diagonal_runs_match_oracle(n, bits):
+    rng = np.random.default_rng(990 + n)
+    ops = diagonal_heavy_circuit(n, rng, 240)
+    init = rand_state(n, rng)
+    psi = po.apply_ops(n, init, ops)
+    bras = obs_states(n, psi, [[("PauliZ", [0])], [("PauliX", [n - 1]), ("PauliY", [3])]])
+    npar = sum(1 for o in ops if o[2])
+    tp = list(range(0, npar, 2))
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    try:
+        N.set_tile_bits(*bits)
+        close128(N.State(n).upload(init).apply(ops).download(), psi)
+        close64(N.State(n, np.complex64).upload(init.astype(np.complex64)).apply(ops).download(), psi)
+        got = N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp)
+        close128(got, ref)
+        got64 = N.adjoint_jacobian(N.State(n, np.complex64).upload(psi.astype(np.complex64)),
+                                   [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
+        close64(got64, ref)
+    finally:
+        N.set_tile_bits(4, 3)
+
+
 def test_fused_adjoint_config2_shape():
     # RY/RZ + CZ brick (the 28-qubit benchmark circuit family) at 14 qubits
     from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
