@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 
@@ -370,6 +371,7 @@ struct lgh_tmpl {
     std::vector<int32_t> pidx;
     std::vector<int32_t> obs_paulis, obs_wires;
     // device copy of the compiled template (uploaded on first use)
+    std::mutex upload_mu;
     void *dev = nullptr;
     size_t dev_bytes = 0;
     ~lgh_tmpl() {
@@ -384,7 +386,11 @@ namespace {
 
 size_t al256(size_t b) { return (b + 255) & ~size_t(255); }
 
-void upload_template(lgh_tmpl *t) {
+// The copy runs on the caller's stream: a copy on the null stream would wait
+// for the work of every other blocking stream of the device (a serving
+// engine's large circuits, for one).
+void upload_template(lgh_tmpl *t, hipStream_t stream) {
+    std::lock_guard<std::mutex> lk(t->upload_mu);
     if (t->dev) return;
     const size_t o_hdr = 0;
     const size_t o_fwd = al256(sizeof(TmplDev));
@@ -417,7 +423,8 @@ void upload_template(lgh_tmpl *t) {
     std::memcpy(h.data() + o_gen, t->gens.data(), sizeof(FGen) * t->gens.size());
     std::memcpy(h.data() + o_slot, t->slots.data(), sizeof(TSlot) * t->slots.size());
     std::memcpy(h.data() + o_rec, t->recipes.data(), sizeof(Recipe) * t->recipes.size());
-    const hipError_t e = hipMemcpy(d, h.data(), total, hipMemcpyHostToDevice);
+    hipError_t e = hipMemcpyAsync(d, h.data(), total, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream); // h is a host temporary
     if (e != hipSuccess) {
         pool_free(t->device, d, total);
         LGH_HIP(e);
@@ -463,7 +470,7 @@ void batch_run_t(lgh_tmpl *const *tmpls, int32_t ncirc, const double *params, co
         StreamCtx *c;
         ~Rel() { release_ctx(c); }
     } rel{ctx};
-    for (int32_t c = 0; c < ncirc; ++c) upload_template(tmpls[c]);
+    for (int32_t c = 0; c < ncirc; ++c) upload_template(tmpls[c], ctx->stream);
     const bool grad = jac != nullptr;
     // layout of one device block: circuit table, parameters, descriptors,
     // jacobian rows, expvals, counter; the states live in a second block

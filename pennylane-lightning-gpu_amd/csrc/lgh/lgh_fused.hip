@@ -343,31 +343,33 @@ __device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *_
         FOp cur = load_op(cops);
         for (int o = 0; o < nops;) {
             if (f_kind(cur) == kDiagRun) {
-                diag_run_adj<R, NR, RB>(cops + o, cur, cgens, base, a, b, acc, threadIdx.x);
+                diag_run_adj<R, NR, RB>(cops + o, cur, base, a, b, acc, threadIdx.x);
                 o += 1 + run_ndata(cur) + run_nmem(cur);
                 cur = load_op(cops + (o < nops ? o : 0));
                 continue;
             }
             const FOp nxt = load_op(cops + (o + 1 < nops ? o + 1 : o));
             const int slot = f_slot(cur);
+            // overlaps go to the lane's own LDS column (no contention; the
+            // additions keep program order)
             if (f_kind(cur) != kDiag && f_plain(cur)) {
                 if (slot >= 0) {
-                    const FGen g = load_gen(cgens + slot);
-                    acc[slot][threadIdx.x] += adj_plain_dense<R, NR, true>(cur, &g, lane, a, b);
+                    // X and Y generators need no generator record
+                    FGen g{};
+                    if (f_gkind(cur) == kGenDense) g = load_gen(cgens + slot);
+                    atomicAdd(&acc[slot][threadIdx.x], adj_plain_dense<R, NR, true>(cur, &g, lane, a, b));
                 } else {
                     adj_plain_dense<R, NR, false>(cur, nullptr, lane, a, b);
                 }
             } else if (diag_fast(cur)) {
-                if (slot >= 0) {
-                    const CFGen *gp = cgens + slot;
-                    acc[slot][threadIdx.x] += adj_diag_step<R, NR, true>(cur, gp->g[0].re, gp->g[1].re, base, a, b);
-                } else {
+                if (slot >= 0)
+                    atomicAdd(&acc[slot][threadIdx.x], adj_diag_step<R, NR, true>(cur, cur.m[2].re, cur.m[3].re, base, a, b));
+                else
                     adj_diag_step<R, NR, false>(cur, 0.0, 0.0, base, a, b);
-                }
             } else {
                 if (slot >= 0) {
-                    const FGen g = load_gen(cgens + slot);
-                    acc[slot][threadIdx.x] += gen_dot<R, NR>(cur, g, lane, base, a, b);
+                    const FGen g = f_kind(cur) == kDiag ? diag_gen(cur) : load_gen(cgens + slot);
+                    atomicAdd(&acc[slot][threadIdx.x], gen_dot<R, NR>(cur, g, lane, base, a, b));
                 }
                 apply_fop<R, NR>(cur, lane, base, a);
                 apply_fop<R, NR>(cur, lane, base, b);
@@ -560,6 +562,8 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
                 h.gkind = kGenDiag;
                 fg.g[0] = cx<double>{g.g_even, 0.0};
                 fg.g[1] = cx<double>{g.g_odd, 0.0};
+                f.m[2] = fg.g[0]; // read in place by the adjoint kernels
+                f.m[3] = fg.g[1];
             } else {
                 h.gkind = kGenDense;
                 for (int q = 0; q < 4; ++q) fg.g[q] = to_cx<double>(g.mat[size_t(q)]);

@@ -649,6 +649,16 @@ __device__ __forceinline__ FGen load_gen(const CFGen *p) {
     return g;
 }
 
+// Trainable diagonal gates carry their generator eigenvalues (even / odd
+// parity) in m[2].re and m[3].re, next to their phases, so the adjoint sweep
+// reads no separate generator record for them.
+__device__ __forceinline__ FGen diag_gen(const FOp &op) {
+    FGen g{};
+    g.g[0].re = op.m[2].re;
+    g.g[1].re = op.m[3].re;
+    return g;
+}
+
 // --- diagonal runs ---------------------------------------------------------
 // per-lane factor of a member without register bits
 template <class R> __device__ __forceinline__ cx<R> run_q(const FOp &m, uint64_t base, cx<R> q) {
@@ -720,8 +730,8 @@ __device__ __forceinline__ void diag_run(const CFOp *hp, const FOp &hdr, uint64_
 // Adjoint: the run applied to lambda (a) and the observable state (b); the
 // overlap of every trainable member is added to its slot of acc.
 template <class R, int NR, int RB>
-__device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, const CFGen *cgens, uint64_t base,
-                                             cx<R> (&a)[NR], cx<R> (&b)[NR], double (*acc)[256], int col) {
+__device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, uint64_t base, cx<R> (&a)[NR],
+                                             cx<R> (&b)[NR], double (*acc)[256], int col) {
     const int nmem = run_nmem(hdr);
     const uint32_t flags = run_flags(hdr);
     const CFOp *mp = hp + 1 + run_ndata(hdr);
@@ -745,14 +755,13 @@ __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, con
         const FOp mn = load_op(mp + (j + 1 < nmem ? j + 1 : j));
         const int slot = f_slot(m);
         if (slot >= 0) {
-            const CFGen *gp = cgens + slot;
             const uint32_t oc = f_oclass(m);
             const bool lane_ok = (base & m.cm_rest) == m.cm_rest;
             double val;
             if (oc == kOvF) {
-                val = diag_dot<R, NR>(m, load_gen(gp), base, lane_ok, a, b);
+                val = diag_dot<R, NR>(m, diag_gen(m), base, lane_ok, a, b);
             } else {
-                const double ge = gp->g[0].re, go = gp->g[1].re;
+                const double ge = m.m[2].re, go = m.m[3].re;
                 const bool lpar = __popcll(base & m.pm_rest) & 1;
                 const double g0 = lpar ? go : ge, g1 = lpar ? ge : go;
                 if (oc == kOvT) {
@@ -766,7 +775,7 @@ __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, con
                 }
                 val = lane_ok ? val : 0.0;
             }
-            acc[slot][col] += val;
+            atomicAdd(&acc[slot][col], val); // own column: no contention, program order
         }
         switch (f_mclass(m)) {
         case kMemQ: q = run_q(m, base, q); break;
