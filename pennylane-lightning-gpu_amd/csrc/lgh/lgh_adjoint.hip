@@ -409,7 +409,12 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
             pool_free(s->device, s->data, s->bytes);
         }
     } free_lam{&lam};
-    LGH_HIP(hipMemcpyAsync(lam.data, psi_in->data, psi_in->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    // The fused sweep reads lambda's starting values from psi itself when no
+    // forward application is asked for; the other forms work on a copy.
+    const bool fused_path = fusion_enabled() && fused_partials_bytes(n, psi_in->device) > 0 && nobs > 0;
+    const bool from_psi = fused_path && !apply_ops;
+    if (!from_psi)
+        LGH_HIP(hipMemcpyAsync(lam.data, psi_in->data, psi_in->bytes, hipMemcpyDeviceToDevice, ctx->stream));
     if (apply_ops)
         for (int64_t i = 0; i < nops; ++i)
             if (!is_prep(ops[i])) launch_kop(&lam, compile_op(ops[i], n, false));
@@ -436,7 +441,7 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
     LGH_HIP(hipMemsetAsync(jac_dev, 0, sizeof(double) * size_t(std::max<int64_t>(1, nobs * ntp)), ctx->stream));
     ctx->sync(); // hp is a host temporary
 
-    if (fusion_enabled() && b_fused > 0 && nobs > 0) {
+    if (fused_path) {
         // Fused reverse sweep, one observable state at a time on a fresh copy
         // of lambda.  Ops before the earliest trainable one are never visited.
         std::vector<KOp> udag;
@@ -465,9 +470,9 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
             }
             slot.push_back(col);
         }
-        lgh_sv lam0 = lam; // forward state every observable sweep starts from
+        lgh_sv lam0 = lam; // forward state every observable sweep starts from (after apply_ops)
         lam0.data = nullptr;
-        if (nobs > 1) {
+        if (nobs > 1 && !from_psi) {
             lam0.data = pool_alloc(psi_in->device, psi_in->bytes);
             LGH_HIP(hipMemcpyAsync(lam0.data, lam.data, lam.bytes, hipMemcpyDeviceToDevice, ctx->stream));
         }
@@ -480,7 +485,7 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
             }
         } free_lam0{&lam0};
         for (int j = 0; j < nobs; ++j) {
-            if (j > 0)
+            if (j > 0 && !from_psi)
                 LGH_HIP(hipMemcpyAsync(lam.data, lam0.data, lam.bytes, hipMemcpyDeviceToDevice, ctx->stream));
             double *jrow = jac_dev + size_t(j) * ntp;
             auto standalone = [&](const KOp &u, const KGen *g, int64_t col, double f) {
@@ -499,7 +504,8 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
                     LGH_LAUNCH_CHECK();
                 }
             };
-            fused_adjoint_one(&lam, bras[j], udag, gens, slot, fac, jrow, fpartials, standalone);
+            fused_adjoint_one(&lam, from_psi ? psi_in->data : nullptr, bras[j], udag, gens, slot, fac, jrow,
+                              fpartials, standalone);
         }
         if (nobs * ntp > 0)
             LGH_HIP(hipMemcpyAsync(jac_host, jac_dev, sizeof(double) * size_t(nobs * ntp), hipMemcpyDeviceToHost,

@@ -284,11 +284,13 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
 #pragma unroll
         for (int k = 0; k < NR; ++k) v[k] = ldnt(p + roff[k]);
         FOp cur = load_op(cops);
+        wait_op(cur);
         for (int o = 0; o < nops;) {
             if (f_kind(cur) == kDiagRun) {
                 diag_run<R, NR>(cops + o, cur, base, v);
                 o += 1 + run_ndata(cur) + run_nmem(cur);
                 cur = load_op(cops + (o < nops ? o : 0));
+                wait_op(cur);
                 continue;
             }
             const FOp nxt = load_op(cops + (o + 1 < nops ? o + 1 : o)); // prefetch the next descriptor
@@ -307,9 +309,13 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
 // ops of the pass are stored in reverse order with U^dagger matrices.  Each
 // lane accumulates its overlap contributions in its own LDS column; the
 // reduction over lanes and wavefronts happens once at the end.
+// lam_in: where lambda is read (lam itself, or the forward state for the
+// first pass of a sweep, which then needs no copy of it); lam: where it is
+// written.  Each thread reads and writes its own amplitudes only, so the two
+// may be the same array.
 template <class R, int RB>
-__device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *__restrict__ bra, uint64_t ntiles,
-                                               uint64_t chunk, const PassDesc *__restrict__ pd,
+__device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
+                                               uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                double *__restrict__ partials) {
     __shared__ double acc[kMaxSlots][kBlock];
@@ -335,17 +341,20 @@ __device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *_
         const uint64_t base = g | loff;
         cx<R> a[NR], b[NR];
         cx<R> *pa = lam + base, *pb = bra + base; // the tile bits are disjoint
+        const cx<R> *qa = lam_in + base;
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
-            a[k] = ldnt(pa + roff[k]);
+            a[k] = ldnt(qa + roff[k]);
             b[k] = ldnt(pb + roff[k]);
         }
         FOp cur = load_op(cops);
+        wait_op(cur);
         for (int o = 0; o < nops;) {
             if (f_kind(cur) == kDiagRun) {
                 diag_run_adj<R, NR, RB>(cops + o, cur, base, a, b, acc, threadIdx.x);
                 o += 1 + run_ndata(cur) + run_nmem(cur);
                 cur = load_op(cops + (o < nops ? o : 0));
+                wait_op(cur);
                 continue;
             }
             const FOp nxt = load_op(cops + (o + 1 < nops ? o + 1 : o));
@@ -396,18 +405,18 @@ __device__ __forceinline__ void fused_adj_body(cx<R> *__restrict__ lam, cx<R> *_
 // Three register bits fit three wavefronts per SIMD (168 VGPRs, 48 KiB LDS
 // per workgroup); four register bits need all registers of one.
 template <class R>
-__global__ __launch_bounds__(kBlock, 3) void k_fused_adj3(cx<R> *__restrict__ lam, cx<R> *__restrict__ bra,
+__global__ __launch_bounds__(kBlock, 3) void k_fused_adj3(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                        uint64_t ntiles, uint64_t chunk,
                                                        const PassDesc *__restrict__ pd, const FOp *__restrict__ ops,
                                                        const FGen *__restrict__ gens, double *__restrict__ partials) {
-    fused_adj_body<R, 3>(lam, bra, ntiles, chunk, pd, ops, gens, partials);
+    fused_adj_body<R, 3>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials);
 }
 template <class R>
-__global__ __launch_bounds__(kBlock) void k_fused_adj4(cx<R> *__restrict__ lam, cx<R> *__restrict__ bra,
+__global__ __launch_bounds__(kBlock) void k_fused_adj4(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                     uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                     const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                     double *__restrict__ partials) {
-    fused_adj_body<R, 4>(lam, bra, ntiles, chunk, pd, ops, gens, partials);
+    fused_adj_body<R, 4>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials);
 }
 
 // slot sums -> jac[col[s]] * factor[s]
@@ -741,9 +750,10 @@ int count_passes(const std::vector<KOp> &kops, int n) {
 }
 
 template <class R>
-static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, const std::vector<KGen> &gens,
-                        const std::vector<int> &slot_col, const std::vector<double> &factor, double *jac_row_dev,
-                        double *partials, StandaloneStep standalone) {
+static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std::vector<KOp> &udag,
+                        const std::vector<KGen> &gens, const std::vector<int> &slot_col,
+                        const std::vector<double> &factor, double *jac_row_dev, double *partials,
+                        StandaloneStep standalone) {
     // `udag` lists the ops in forward order with U^dagger matrices; the passes
     // are scheduled on that order and executed backwards.
     StreamCtx *ctx = lam->ctx;
@@ -751,9 +761,15 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
     const int rb = g_rb_adj.load();
     const std::vector<Pass> passes = schedule_passes(udag, n, slot_col, rb, fixed_lanes(lam->dtype));
     const Geometry geo = geometry(n, lam->device, rb);
+    // lambda still to be read from lam_src (until the first pass has run)
+    const void *src = lam_src;
     for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
         const Pass &p = *it;
         if (p.standalone) {
+            if (src) {
+                LGH_HIP(hipMemcpyAsync(lam->data, src, lam->bytes, hipMemcpyDeviceToDevice, ctx->stream));
+                src = nullptr;
+            }
             const int i = p.ops[0];
             standalone(udag[size_t(i)], slot_col[size_t(i)] >= 0 ? &gens[size_t(i)] : nullptr, slot_col[size_t(i)],
                        factor[size_t(i)]);
@@ -781,7 +797,9 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
         {
             ProfScope prof(kProfFusedAdj, ctx->stream);
             auto kern = rb == 4 ? k_fused_adj4<R> : k_fused_adj3<R>;
-            hipLaunchKernelGGL(kern, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream,
+            const cx<R> *lam_in = static_cast<const cx<R> *>(src ? src : lam->data);
+            src = nullptr;
+            hipLaunchKernelGGL(kern, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream, lam_in,
                                static_cast<cx<R> *>(lam->data), static_cast<cx<R> *>(bra->data), geo.ntiles,
                                geo.chunk, reinterpret_cast<const PassDesc *>(d), reinterpret_cast<const FOp *>(d + pb),
                                reinterpret_cast<const FGen *>(d + pb + ob), partials);
@@ -796,13 +814,14 @@ static void adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, 
     }
 }
 
-void fused_adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, const std::vector<KGen> &gens,
-                       const std::vector<int> &slot_col, const std::vector<double> &factor, double *jac_row_dev,
-                       double *partials, StandaloneStep standalone) {
+void fused_adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std::vector<KOp> &udag,
+                       const std::vector<KGen> &gens, const std::vector<int> &slot_col,
+                       const std::vector<double> &factor, double *jac_row_dev, double *partials,
+                       StandaloneStep standalone) {
     if (lam->dtype == LGH_C128)
-        adjoint_one<double>(lam, bra, udag, gens, slot_col, factor, jac_row_dev, partials, standalone);
+        adjoint_one<double>(lam, lam_src, bra, udag, gens, slot_col, factor, jac_row_dev, partials, standalone);
     else
-        adjoint_one<float>(lam, bra, udag, gens, slot_col, factor, jac_row_dev, partials, standalone);
+        adjoint_one<float>(lam, lam_src, bra, udag, gens, slot_col, factor, jac_row_dev, partials, standalone);
 }
 
 size_t fused_partials_bytes(int n, int device) {

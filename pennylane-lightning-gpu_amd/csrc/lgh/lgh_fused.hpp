@@ -47,10 +47,13 @@ using StandaloneStep = std::function<void(const KOp &udag, const KGen *gen, int6
 
 // Reverse sweep of one observable state through fused passes.  udag: the
 // ops in forward order compiled as U^dagger; slot_col[i]: Jacobian column of
-// a trainable gate or -1; factor[i] = -2 s (inverse ? -1 : 1).
-void fused_adjoint_one(lgh_sv *lam, lgh_sv *bra, const std::vector<KOp> &udag, const std::vector<KGen> &gens,
-                       const std::vector<int> &slot_col, const std::vector<double> &factor, double *jac_row_dev,
-                       double *partials, StandaloneStep standalone);
+// a trainable gate or -1; factor[i] = -2 s (inverse ? -1 : 1).  lam_src: if
+// not null, lambda's starting values, read by the first pass (or copied
+// before a first standalone step) instead of lam's own contents.
+void fused_adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std::vector<KOp> &udag,
+                       const std::vector<KGen> &gens, const std::vector<int> &slot_col,
+                       const std::vector<double> &factor, double *jac_row_dev, double *partials,
+                       StandaloneStep standalone);
 size_t fused_partials_bytes(int n, int device);
 
 } // namespace lgh

@@ -642,6 +642,15 @@ __device__ __forceinline__ FOp load_op(const CFOp *p) {
     for (int q = 0; q < 4; ++q) f.m[q] = cx<double>{p->m[q].re, p->m[q].im};
     return f;
 }
+// Makes the compiler wait for the descriptor op (an empty asm reading its
+// first and last dwords).  Called on a descriptor loaded before a loop:
+// without it the wait for that load lands at the loop header, where it also
+// waits for the loop's own one-ahead prefetch, so the prefetch would hide
+// nothing.
+__device__ __forceinline__ void wait_op(const FOp &op) {
+    asm volatile("" ::"s"(op.meta), "s"(op.rbits), "s"(__double2hiint(op.m[3].im)), "s"(op.pad));
+}
+
 __device__ __forceinline__ FGen load_gen(const CFGen *p) {
     FGen g;
 #pragma unroll
@@ -714,6 +723,7 @@ __device__ __forceinline__ void diag_run(const CFOp *hp, const FOp &hdr, uint64_
     cx<R> q{R(1), R(0)};
     uint32_t sm = 0;
     FOp m = load_op(mp);
+    wait_op(m);
     for (int j = 0; j < nmem; ++j) {
         const FOp mn = load_op(mp + (j + 1 < nmem ? j + 1 : j));
         switch (f_mclass(m)) {
@@ -751,6 +761,7 @@ __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, uin
     cx<R> q{R(1), R(0)};
     uint32_t sm = 0;
     FOp m = load_op(mp);
+    wait_op(m);
     for (int j = 0; j < nmem; ++j) {
         const FOp mn = load_op(mp + (j + 1 < nmem ? j + 1 : j));
         const int slot = f_slot(m);
