@@ -183,6 +183,14 @@ lgh_status lgh_inner(const lgh_sv *a, const lgh_sv *b, double *re, double *im);
 lgh_status lgh_axpy(lgh_sv *y, double alpha_re, double alpha_im, const lgh_sv *x);
 /* scaleC_CUDA: x *= alpha */
 lgh_status lgh_scale(lgh_sv *x, double alpha_re, double alpha_im);
+/* dst = P src for the Pauli word P = paulis[k] ('I', 'X', 'Y', 'Z') on
+ * wires[k]; dst and src are different states of the same size.  One pass.
+ * It replaces the reference's copy of the state followed by the observable's
+ * gates when the adjoint Jacobian forms H|lambda> for a Pauli-word
+ * observable (AdjointDiffGPU.hpp:228-268, ObservablesGPU.hpp:88-128,
+ * :209-260); no single reference function corresponds to it. */
+lgh_status lgh_sv_pauli_word_from(lgh_sv *dst, const lgh_sv *src, const char *paulis, const int32_t *wires,
+                                  int32_t num_wires);
 
 /* ---- measurements (SVCM:702-1148) ------------------------------------ */
 /* expval(wires, matrix) / getExpectationValueDeviceMatrix (SVCM:755-780, :1649-1713) */

@@ -97,8 +97,9 @@ template <class T> class AdjointJacobianHIP {
         std::vector<StateVectorHIPManaged<T>> bras;
         bras.reserve(obs.size());
         for (const auto &o : obs) {
-            bras.emplace_back(lambda);
-            o->applyInPlace(bras.back());
+            bras.emplace_back(lambda.getNumQubits(), lambda.getDevTag(),
+                              typename StateVectorHIPManaged<T>::Uninitialized{});
+            o->applyFrom(lambda, bras.back());
         }
         std::vector<lgh_sv *> handles;
         for (auto &s : bras) handles.push_back(s.handle());

@@ -33,6 +33,25 @@ template <class T> class ObservableHIP {
   public:
     virtual ~ObservableHIP() = default;
     virtual void applyInPlace(StateVectorHIPManaged<T> &sv) const = 0;
+    // Appends the observable as a Pauli word (letters I/X/Y/Z and their
+    // wires) and returns true when it is one; then O|s> can be formed from a
+    // source state in one pass (StateVectorHIPManaged::setPauliWordOf).
+    virtual bool pauliWord(std::string &letters, std::vector<std::size_t> &wires) const {
+        (void)letters;
+        (void)wires;
+        return false;
+    }
+    // out <- O src (out: any contents, same size as src).
+    void applyFrom(const StateVectorHIPManaged<T> &src, StateVectorHIPManaged<T> &out) const {
+        std::string letters;
+        std::vector<std::size_t> wires;
+        if (pauliWord(letters, wires)) {
+            out.setPauliWordOf(src, letters, wires);
+            return;
+        }
+        out.updateData(src, true);
+        applyInPlace(out);
+    }
     virtual std::string getObsName() const = 0;
     virtual std::vector<std::size_t> getWires() const = 0;
     bool operator==(const ObservableHIP &other) const {
@@ -51,6 +70,14 @@ template <class T> class NamedObsHIP final : public ObservableHIP<T> {
         : name_(std::move(name)), wires_(std::move(wires)), params_(std::move(params)) {}
     void applyInPlace(StateVectorHIPManaged<T> &sv) const override {
         sv.applyOperation(name_, wires_, false, params_);
+    }
+    bool pauliWord(std::string &letters, std::vector<std::size_t> &wires) const override {
+        if (name_ == "Identity") return true;
+        const char c = name_ == "PauliX" ? 'X' : name_ == "PauliY" ? 'Y' : name_ == "PauliZ" ? 'Z' : 0;
+        if (!c || wires_.size() != 1) return false;
+        letters.push_back(c);
+        wires.push_back(wires_[0]);
+        return true;
     }
     std::string getObsName() const override { return name_ + wiresToString(wires_); }
     std::vector<std::size_t> getWires() const override { return wires_; }
@@ -119,6 +146,11 @@ template <class T> class TensorProdObsHIP final : public ObservableHIP<T> {
     void applyInPlace(StateVectorHIPManaged<T> &sv) const override {
         for (const auto &o : obs_) o->applyInPlace(sv);
     }
+    bool pauliWord(std::string &letters, std::vector<std::size_t> &wires) const override {
+        for (const auto &o : obs_)
+            if (!o->pauliWord(letters, wires)) return false;
+        return true;
+    }
     std::string getObsName() const override {
         std::string s;
         for (std::size_t i = 0; i < obs_.size(); ++i) {
@@ -155,8 +187,9 @@ template <class T> class HamiltonianHIP final : public ObservableHIP<T> {
         StateVectorHIPManaged<T> acc(sv.getNumQubits(), sv.getDevTag());
         acc.scale({T(0), T(0)});
         for (std::size_t t = 0; t < obs_.size(); ++t) {
-            StateVectorHIPManaged<T> tmp(sv);
-            obs_[t]->applyInPlace(tmp);
+            StateVectorHIPManaged<T> tmp(sv.getNumQubits(), sv.getDevTag(),
+                                         typename StateVectorHIPManaged<T>::Uninitialized{});
+            obs_[t]->applyFrom(sv, tmp);
             acc.axpy({coeffs_[t], T(0)}, tmp);
         }
         sv.updateData(acc);

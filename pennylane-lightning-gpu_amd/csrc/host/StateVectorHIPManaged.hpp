@@ -194,6 +194,12 @@ template <class PrecisionT> class StateVectorHIPManaged {
         num_qubits_ = num_qubits;
         initSV();
     }
+    // A state whose amplitudes the caller overwrites next (no |0...0> fill).
+    struct Uninitialized {};
+    StateVectorHIPManaged(std::size_t num_qubits, const DevTag &tag, Uninitialized) : tag_(tag) {
+        check(lgh_sv_create(&sv_, static_cast<int>(num_qubits), kDtype, tag.getDeviceID(), tag.getStreamID()));
+        num_qubits_ = num_qubits;
+    }
     StateVectorHIPManaged(const ComplexT *data, std::size_t length, const DevTag &tag = DevTag{}) : tag_(tag) {
         PL_HIP_ABORT_IF(length == 0 || (length & (length - 1)) != 0, "State length must be a power of two");
         std::size_t n = 0;
@@ -259,6 +265,12 @@ template <class PrecisionT> class StateVectorHIPManaged {
         if (!async) check(lgh_sv_synchronize(sv_));
     }
     void synchronize() const { check(lgh_sv_synchronize(sv_)); }
+    // this <- P other for a Pauli word (letters I/X/Y/Z on wires), one pass
+    void setPauliWordOf(const StateVectorHIPManaged &other, const std::string &letters,
+                        const std::vector<std::size_t> &wires) {
+        std::vector<int32_t> w(wires.begin(), wires.end());
+        check(lgh_sv_pauli_word_from(sv_, other.sv_, letters.data(), w.data(), static_cast<int32_t>(w.size())));
+    }
 
     // ---- gates (SVCM:198-315)
     void applyOperation(const std::string &opName, const std::vector<std::size_t> &wires, bool adjoint = false,

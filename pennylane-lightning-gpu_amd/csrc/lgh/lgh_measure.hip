@@ -138,6 +138,24 @@ __global__ __launch_bounds__(kBlock) void k_axpy(cx<R> *__restrict__ y, const cx
     if (i < len) y[i] = cfma(a, x[i], y[i]);
 }
 
+// dst = P src for the Pauli word P with X/Y bits xmask, Y/Z bits zmask and
+// ny Y factors: (P src)[i] = i^ny (-1)^popcount(j & zmask) src[j], j = i ^ xmask.
+// One read and one write per amplitude; the reads of a wavefront stay in the
+// same cache lines as its writes when xmask has only low bits, and are whole
+// lines otherwise.
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_pauli_from(cx<R> *__restrict__ dst, const cx<R> *__restrict__ src,
+                                                       uint64_t len, uint64_t xmask, uint64_t zmask, int ny) {
+    const uint64_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= len) return;
+    const uint64_t j = i ^ xmask;
+    cx<R> v = ldnt(src + j);
+    if (ny & 2) v = cx<R>{-v.re, -v.im};       // i^2 = -1
+    if (ny & 1) v = cx<R>{-v.im, v.re};        // times i
+    if (__popcll(j & zmask) & 1) v = cx<R>{-v.re, -v.im};
+    stnt(dst + i, v);
+}
+
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_scale(cx<R> *__restrict__ x, uint64_t len, cx<R> a) {
     const uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
@@ -821,6 +839,43 @@ lgh_status lgh_axpy(lgh_sv *y, double are, double aim, const lgh_sv *x) {
             using R = decltype(r);
             hipLaunchKernelGGL(k_axpy<R>, dim3(blocks), dim3(kBlock), 0, y->ctx->stream, mdata<R>(y), cdata<R>(x),
                                y->length(), cx<R>{R(are), R(aim)});
+        });
+        LGH_LAUNCH_CHECK();
+    });
+}
+
+lgh_status lgh_sv_pauli_word_from(lgh_sv *dst, const lgh_sv *src, const char *paulis, const int32_t *wires,
+                                  int32_t num_wires) {
+    return guard([&] {
+        LGH_REQUIRE(dst && src, "null state");
+        LGH_REQUIRE(dst != src && dst->data != src->data, "source and destination must differ");
+        LGH_REQUIRE(dst->n == src->n && dst->dtype == src->dtype, "states differ in size or precision");
+        LGH_REQUIRE(num_wires >= 0 && (num_wires == 0 || (paulis && wires)), "null Pauli word");
+        const int n = dst->n;
+        uint64_t xmask = 0, zmask = 0, seen = 0;
+        int ny = 0;
+        for (int32_t k = 0; k < num_wires; ++k) {
+            LGH_REQUIRE(wires[k] >= 0 && wires[k] < n, "wire out of range");
+            const uint64_t bit = 1ull << (n - 1 - wires[k]);
+            LGH_REQUIRE(!(seen & bit), "repeated wire in a Pauli word");
+            seen |= bit;
+            switch (paulis[k]) {
+            case 'I': break;
+            case 'X': xmask |= bit; break;
+            case 'Y': xmask |= bit; zmask |= bit; ++ny; break;
+            case 'Z': zmask |= bit; break;
+            default: throw Error(LGH_ERR_INVALID_ARGUMENT, "Pauli word letters are I, X, Y, Z");
+            }
+        }
+        DeviceGuard g(dst->device);
+        if (src->ctx != dst->ctx) src->ctx->sync();
+        const uint64_t len = dst->length();
+        unsigned blocks = unsigned((len + kBlock - 1) / kBlock);
+        if (blocks >= 8) blocks = (blocks + 7) / 8 * 8; // whole XCD rounds for the block order
+        by_dtype(dst, [&](auto r) {
+            using R = decltype(r);
+            hipLaunchKernelGGL(k_pauli_from<R>, dim3(blocks), dim3(kBlock), 0, dst->ctx->stream, mdata<R>(dst),
+                               cdata<R>(src), len, xmask, zmask, ny & 3);
         });
         LGH_LAUNCH_CHECK();
     });

@@ -88,6 +88,7 @@ def lib():
         L.lgh_apply_generator.argtypes = [vp, C.POINTER(Op), C.POINTER(C.c_double)]
         L.lgh_inner.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.lgh_axpy.argtypes = [vp, C.c_double, C.c_double, vp]
+        L.lgh_sv_pauli_word_from.argtypes = [vp, vp, C.c_char_p, vp, C.c_int32]
         L.lgh_scale.argtypes = [vp, C.c_double, C.c_double]
         L.lgh_expval_matrix.argtypes = [vp, vp, C.c_int32, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.lgh_expval_pauli_words.argtypes = [vp, C.c_int64, C.c_char_p, vp, vp, vp, vp, C.POINTER(C.c_double)]
@@ -281,6 +282,14 @@ class State:
     def axpy(self, alpha, x: "State"):
         a = complex(alpha)
         check(lib().lgh_axpy(self.handle, a.real, a.imag, x.handle))
+
+    def set_pauli_word_of(self, src: "State", word: str, wires):
+        """self <- P src for the Pauli word `word` (letters I/X/Y/Z) on `wires`; one pass."""
+        w = np.ascontiguousarray(wires, dtype=np.int32)
+        if len(word) != len(w):
+            raise ValueError("one wire per Pauli letter")
+        check(lib().lgh_sv_pauli_word_from(self.handle, src.handle, word.encode(), w.ctypes.data, len(w)))
+        return self
 
     def scale(self, alpha):
         a = complex(alpha)

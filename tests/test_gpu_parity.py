@@ -237,6 +237,27 @@ def test_sampler_bit_exact(n):
     np.testing.assert_array_equal(bits, expect)
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_pauli_word_from(dtype):
+    """dst = P src in one pass against the oracle's gate-by-gate P src, for
+    words with every letter, low and high wires, and an odd number of Y."""
+    rng = np.random.default_rng(61)
+    n = 12
+    psi = rand_state(n, rng)
+    src = N.State(n, dtype).upload(psi.astype(dtype))
+    for word, wires in (("Z", [0]), ("X", [n - 1]), ("Y", [5]), ("ZZ", [0, n - 1]), ("XYZI", [11, 3, 7, 0]),
+                        ("YYY", [1, 10, 6]), ("XX", [2, 9]), ("", [])):
+        dst = N.State(n, dtype).set_pauli_word_of(src, word, wires)
+        ops = [({"X": "PauliX", "Y": "PauliY", "Z": "PauliZ"}[c], [w], [], False) for c, w in zip(word, wires)
+               if c != "I"]
+        ref = po.apply_ops(n, psi, ops)
+        (close128 if dtype == np.complex128 else close64)(dst.download(), ref)
+    with pytest.raises(N.LghError):
+        src.set_pauli_word_of(src, "Z", [0])
+    with pytest.raises(N.LghError):
+        N.State(n, dtype).set_pauli_word_of(src, "ZZ", [3, 3])
+
+
 def test_inner_axpy_scale():
     rng = np.random.default_rng(16)
     n = 10
