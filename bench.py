@@ -161,10 +161,14 @@ def bench_batched(N, args):
 
 def bench_measurements(N, args):
     """Measurement workloads, complex128: the state of 8 x [RX, RY, RZ on every
-    wire; CNOT ladder] from |0..0>.  (1) <H> of 256 random Pauli words (1-4
-    factors) on 24 qubits; (2) <H> of the open-chain TFIM (J = 1, h = 0.7) as
-    a resident CSR on 20 qubits; (3) 10^6 seeded samples on 24 qubits.  Times
-    are HIP events on the state's stream, median of repeats."""
+    wire; CNOT ladder] from |0..0>.  On 24 qubits: <H> of 256 random Pauli
+    words (1-4 factors); a dense 2-wire expectation <Z0 Z23>; the variance of
+    a random 3-wire Hermitian (two dense expectations, as the reference
+    computes it); marginal probabilities of 10 wires; 10^6 seeded samples.  On
+    20 qubits: <H> of the open-chain TFIM (J = 1, h = 0.7) as a resident CSR.
+    Times are HIP events on the state's stream, median of repeats.  Last, the
+    torch.autograd form: forward and backward of 256 circuits of the 12-qubit
+    ring template (host wall clock)."""
     import numpy as np
 
     from pennylane_lightning_gpu.workloads import ladder, random_pauli_hamiltonian, tfim_csr
@@ -187,6 +191,29 @@ def bench_measurements(N, args):
     out["pauli_words"] = {"qubits": n, "words": len(words), "ms": round(ms, 3), "passes": passes,
                           "bytes": passes * S, "GBps": round(passes * S / (ms * 1e-3) / 1e9, 1),
                           "words_per_s": round(len(words) / (ms * 1e-3), 1)}
+    # dense expectation <Z0 Z23> and a 3-wire Hermitian (one read pass each), and
+    # the variance of the latter as the reference computes it (two passes)
+    Z = np.diag([1.0, -1.0])
+    zz = np.kron(Z, Z).astype(np.complex128)
+    psi.expval_matrix([0, n - 1], zz)  # warm
+    ms = sorted(timer.time(lambda: psi.expval_matrix([0, n - 1], zz)) for _ in range(5))[2]
+    out["expval_dense"] = {"qubits": n, "wires": 2, "ms": round(ms, 3), "bytes": S,
+                           "GBps": round(S / (ms * 1e-3) / 1e9, 1)}
+    rng = np.random.default_rng(args.seed)
+    A = rng.normal(size=(8, 8)) + 1j * rng.normal(size=(8, 8))
+    Hm = (A + A.conj().T) / 2
+    Hm2 = Hm @ Hm
+    w3 = [2, 11, 19]
+    psi.expval_matrix(w3, Hm)  # warm
+    ms_var = sorted(timer.time(lambda: (psi.expval_matrix(w3, Hm), psi.expval_matrix(w3, Hm2)))
+                    for _ in range(5))[2]
+    out["variance_hermitian3"] = {"qubits": n, "wires": 3, "ms": round(ms_var, 3), "bytes": 2 * S,
+                                  "GBps": round(2 * S / (ms_var * 1e-3) / 1e9, 1)}
+    pw = list(range(0, n, 2))[:10]
+    psi.probs(pw)  # warm
+    ms = sorted(timer.time(lambda: psi.probs(pw)) for _ in range(5))[2]
+    out["probabilities"] = {"qubits": n, "wires": len(pw), "ms": round(ms, 3), "bytes": S,
+                            "GBps": round(S / (ms * 1e-3) / 1e9, 1)}
     shots = 10 ** 6
     psi.sample(shots, args.seed)  # warm
     ms = sorted(timer.time(lambda: psi.sample(shots, args.seed)) for _ in range(3))[1]
@@ -206,6 +233,31 @@ def bench_measurements(N, args):
     cbytes = 24 * nnz + 8 * (NN + 1) + 16 * NN
     out["csr_tfim"] = {"qubits": n, "nnz": nnz, "ms": round(ms, 3), "bytes": cbytes,
                        "GBps": round(cbytes / (ms * 1e-3) / 1e9, 1)}
+    del st, csr
+    # torch.autograd over a batch: 256 circuits of the 12-qubit ring template,
+    # forward (one batched launch with the Jacobians) and backward
+    import time
+
+    import torch
+
+    from pennylane_lightning_gpu.torch_layer import batched_expval
+    from pennylane_lightning_gpu.workloads import ring_template
+
+    ops, pidx = ring_template(12)
+    t = N.Template(12, ops, pidx, [("Z", 0)])
+    p = torch.tensor(np.random.default_rng(args.seed).uniform(0, 2 * np.pi, (256, t.num_params)),
+                     dtype=torch.float64, requires_grad=True)
+    batched_expval(p, t).sum().backward()  # warm
+    times = []
+    for _ in range(5):
+        p.grad = None
+        t0 = time.perf_counter()
+        batched_expval(p, t).sum().backward()
+        times.append(time.perf_counter() - t0)
+    ms = sorted(times)[2] * 1e3
+    out["torch_batched"] = {"qubits": 12, "circuits": 256, "params": t.num_params, "ms_fwd_bwd": round(ms, 3),
+                            "circuits_per_s": round(256 / (ms * 1e-3), 1)}
+    t.close()
     return out
 
 
