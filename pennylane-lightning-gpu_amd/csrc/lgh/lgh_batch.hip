@@ -41,6 +41,9 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
+// lane positions 0-3 stay on index bits 0-3 (runs of 256 B for complex128,
+// 128 B for complex64); the forward and reverse passes choose the other two
+constexpr int kFixedLanes = 4;
 constexpr int kMaxBatchQubits = 20;
 // One kernel holds the forward, observable and adjoint passes; three
 // register bits keep its register use low enough for two wavefronts per SIMD.
@@ -174,26 +177,46 @@ __device__ __forceinline__ void store_tile(cx<R> *p, const uint64_t (&roff)[NR],
     for (int k = 0; k < NR; ++k) stnt(p + roff[k], v[k]);
 }
 
+// Tile geometry of one pass: register offsets, all tile bits, and this
+// lane's index offset (lane positions carry the pass's lane bits).  A
+// wavefront visits tiles w, w + kWaves, ...: tile_start gives the first
+// index of tile w, tile_step moves kWaves tiles on.
 struct PassGeo {
     uint64_t roff[NR];
-    int rs[RB];
+    uint64_t tmask;
+    uint64_t loff;
 };
-__device__ __forceinline__ PassGeo pass_geo(const CTPass *tp) {
+__device__ __forceinline__ PassGeo pass_geo(const CTPass *tp, int lane) {
     PassGeo g;
 #pragma unroll
     for (int k = 0; k < NR; ++k) g.roff[k] = tp->pd.reg_off[k];
+    g.tmask = tp->pd.tile_mask;
+    g.loff = 0;
 #pragma unroll
-    for (int r = 0; r < RB; ++r) g.rs[r] = tp->pd.reg_sorted[r];
+    for (int q = 0; q < kLaneBits; ++q)
+        if ((lane >> q) & 1) g.loff |= 1ull << tp->pd.lane_bits[q];
+    return g;
+}
+__device__ __forceinline__ uint64_t tile_start(const CTPass *tp, uint64_t tile) {
+    int fs[kMaxFree];
+#pragma unroll
+    for (int r = 0; r < kMaxFree; ++r) fs[r] = tp->pd.free_sorted[r];
+    return tile_base_free(tile, tp->pd.nfix, fs, tp->pd.nfree);
+}
+__device__ __forceinline__ uint64_t tile_step(uint64_t g, uint64_t tmask) {
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) g = next_tile(g, tmask);
     return g;
 }
 
 template <class R>
 __device__ __forceinline__ void fwd_pass(const CTPass *tp, const CFOp *fops, cx<R> *psi, uint64_t ntiles, int w, int lane) {
-    const PassGeo g = pass_geo(tp);
+    const PassGeo g = pass_geo(tp, lane);
     const int nops = tp->nops;
     const CFOp *ops = fops + tp->op_begin;
-    for (uint64_t tile = w; tile < ntiles; tile += kWaves) {
-        const uint64_t base = tile_base<RB>(tile, g.rs) | uint64_t(lane);
+    uint64_t gt = tile_start(tp, uint64_t(w));
+    for (uint64_t tile = w; tile < ntiles; tile += kWaves, gt = tile_step(gt, g.tmask)) {
+        const uint64_t base = gt | g.loff;
         cx<R> *p = psi + base;
         cx<R> v[NR];
         load_tile(p, g.roff, v);
@@ -214,12 +237,13 @@ __device__ __forceinline__ void fwd_pass(const CTPass *tp, const CFOp *fops, cx<
 template <class R>
 __device__ __forceinline__ double obs_pass(const CTPass *tp, const CFOp *fops, const cx<R> *psi, cx<R> *bra, uint64_t ntiles, int w,
                            int lane) {
-    const PassGeo g = pass_geo(tp);
+    const PassGeo g = pass_geo(tp, lane);
     const int nops = tp->nops;
     const CFOp *ops = fops + tp->op_begin;
     double e = 0.0;
-    for (uint64_t tile = w; tile < ntiles; tile += kWaves) {
-        const uint64_t base = tile_base<RB>(tile, g.rs) | uint64_t(lane);
+    uint64_t gt = tile_start(tp, uint64_t(w));
+    for (uint64_t tile = w; tile < ntiles; tile += kWaves, gt = tile_step(gt, g.tmask)) {
+        const uint64_t base = gt | g.loff;
         cx<R> v[NR];
         load_tile(psi + base, g.roff, v);
         cx<R> u[NR];
@@ -236,12 +260,13 @@ __device__ __forceinline__ double obs_pass(const CTPass *tp, const CFOp *fops, c
 template <class R>
 __device__ __forceinline__ void rev_pass(const CTPass *tp, const CFOp *fops, const CFGen *gens, cx<R> *lam, cx<R> *bra,
                          uint64_t ntiles, int w, int lane, double (*acc)[kBlock]) {
-    const PassGeo g = pass_geo(tp);
+    const PassGeo g = pass_geo(tp, lane);
     const int nops = tp->nops;
     const CFOp *ops = fops + tp->op_begin;
     const int tid = threadIdx.x;
-    for (uint64_t tile = w; tile < ntiles; tile += kWaves) {
-        const uint64_t base = tile_base<RB>(tile, g.rs) | uint64_t(lane);
+    uint64_t gt = tile_start(tp, uint64_t(w));
+    for (uint64_t tile = w; tile < ntiles; tile += kWaves, gt = tile_step(gt, g.tmask)) {
+        const uint64_t base = gt | g.loff;
         cx<R> *pa = lam + base, *pb = bra + base;
         cx<R> a[NR], b[NR];
         load_tile(pa, g.roff, a);
@@ -291,7 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const CircDev *__restrict__ ci
     __shared__ double red[kMaxSlots];
     __shared__ double red_e[kWaves];
     __shared__ int s_c;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const CCirc *cc = (const CCirc *)circs;
     for (;;) {
         if (tid == 0) s_c = int(atomicAdd(counter, 1u));
@@ -653,7 +678,7 @@ lgh_status lgh_tmpl_create(lgh_tmpl **out, int32_t num_qubits, const lgh_op *ops
         };
         int nf = 0;
         {
-            const std::vector<Pass> passes = schedule_passes(fk, nq, {}, RB);
+            const std::vector<Pass> passes = schedule_passes(fk, nq, {}, RB, kFixedLanes);
             for (const Pass &p : passes) {
                 LGH_REQUIRE(!p.standalone, "gate does not fit a fused pass");
                 Built b = build_pass(p, fk, nq, false, nullptr, nullptr, RB, false);
@@ -687,7 +712,7 @@ lgh_status lgh_tmpl_create(lgh_tmpl **out, int32_t num_qubits, const lgh_op *ops
                     slot_of.push_back(-1);
                 }
             }
-            const std::vector<Pass> passes = schedule_passes(uk, nq, slot_of, RB);
+            const std::vector<Pass> passes = schedule_passes(uk, nq, slot_of, RB, kFixedLanes);
             for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
                 const Pass &p = *it;
                 LGH_REQUIRE(!p.standalone, "gate does not fit a fused pass");
