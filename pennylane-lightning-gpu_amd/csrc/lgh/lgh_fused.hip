@@ -698,11 +698,12 @@ struct Geometry {
 // Wavefronts per CU of a pass's grid: many more than fit at once (12 per CU
 // for the default kernels), each taking a short run of consecutive tiles, so
 // that the dispatcher keeps every CU fed to the end of the pass.  Measured at
-// 28 qubits: 192 per CU (11 tiles each) beats 32 (64 tiles each) by 6 % for
-// the adjoint pass, whose per-workgroup overlap reduction makes still
-// smaller runs cost more; the forward pass gains up to 384 (3 tiles each).
-constexpr int kWavesPerCUForward = 384;
-constexpr int kWavesPerCUAdjoint = 192;
+// 28 qubits: 192-256 per CU (8-11 tiles each) beat 32 (64 tiles each) by
+// 6-7 % for the adjoint pass, whose per-workgroup overlap reduction makes
+// still shorter runs cost more; the forward pass is fastest with one tile
+// per wavefront (5 % over 32 tiles).
+constexpr int kWavesPerCUForward = 4096; // one tile per wavefront up to 30 qubits
+constexpr int kWavesPerCUAdjoint = 256;
 
 Geometry geometry(int n, int device, int rb, int waves_per_cu) {
     Geometry g{};
