@@ -470,6 +470,68 @@ bool is_neg_diag(const KOp &k) {
     return k.kind == KOp::DIAG && std::abs(k.d_even + 1.0) < 1e-300 && std::abs(k.d_odd + 1.0) < 1e-300;
 }
 
+// Reorders the gates of a pass (`order`, a valid execution order) so that the
+// diagonal gates come in as few groups as possible: take every diagonal gate
+// whose predecessors are placed, then every other gate whose predecessors are
+// placed, and repeat.  Only commuting gates trade places (two gates commute
+// when every qubit they share is used diagonally by both), so the pass's
+// product is unchanged, and so is every adjoint overlap: a gate's derivative
+// commutes with whatever the gate commutes with.  Consecutive diagonal gates
+// become one diagonal run, and single-qubit gates on one bit stay mergeable
+// (nothing that touches the bit moves between them).
+void cluster_diagonals(std::vector<int> &order, const std::vector<KOp> &kops, int n) {
+    const int m = static_cast<int>(order.size());
+    if (m < 3) return;
+    std::vector<Use> use(size_t(m), Use{});
+    std::vector<char> diag(static_cast<size_t>(m));
+    for (int i = 0; i < m; ++i) {
+        use[size_t(i)] = uses_of(kops[size_t(order[size_t(i)])]);
+        diag[size_t(i)] = kops[size_t(order[size_t(i)])].kind == KOp::DIAG;
+    }
+    // dependency edges as in schedule_passes: per bit, the last non-diagonal
+    // user and the diagonal users since
+    std::vector<std::vector<int>> succ(static_cast<size_t>(m));
+    std::vector<int> indeg(size_t(m), 0), last_nd(size_t(n), -1), deps;
+    std::vector<std::vector<int>> diag_since(static_cast<size_t>(n));
+    for (int j = 0; j < m; ++j) {
+        deps.clear();
+        for (int b = 0; b < n; ++b) {
+            const uint64_t bit = 1ull << b;
+            if (use[size_t(j)].nondiag & bit) {
+                if (last_nd[size_t(b)] >= 0) deps.push_back(last_nd[size_t(b)]);
+                deps.insert(deps.end(), diag_since[size_t(b)].begin(), diag_since[size_t(b)].end());
+                last_nd[size_t(b)] = j;
+                diag_since[size_t(b)].clear();
+            } else if (use[size_t(j)].diag & bit) {
+                if (last_nd[size_t(b)] >= 0) deps.push_back(last_nd[size_t(b)]);
+                diag_since[size_t(b)].push_back(j);
+            }
+        }
+        std::sort(deps.begin(), deps.end());
+        deps.erase(std::unique(deps.begin(), deps.end()), deps.end());
+        for (int d : deps) {
+            succ[size_t(d)].push_back(j);
+            ++indeg[size_t(j)];
+        }
+    }
+    std::vector<int> out;
+    out.reserve(size_t(m));
+    std::vector<char> placed(size_t(m), 0);
+    bool want_diag = diag[0] != 0;
+    while (static_cast<int>(out.size()) < m) {
+        // one sweep in index order places a gate's successors later in the
+        // same sweep once they are ready, so the sweep reaches the closure
+        for (int i = 0; i < m; ++i) {
+            if (placed[size_t(i)] || indeg[size_t(i)] != 0 || (diag[size_t(i)] != 0) != want_diag) continue;
+            placed[size_t(i)] = 1;
+            out.push_back(order[size_t(i)]);
+            for (int s : succ[size_t(i)]) --indeg[size_t(s)];
+        }
+        want_diag = !want_diag;
+    }
+    order.swap(out);
+}
+
 } // namespace
 
 Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse,
@@ -522,6 +584,7 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
     };
     std::vector<int> order = p.ops;
     if (reverse) std::reverse(order.begin(), order.end());
+    if (diag_runs) cluster_diagonals(order, kops, n);
     // forward passes merge consecutive single-qubit gates on the same tile bit
     // into one 2x2 (untrained passes only)
     const bool merge = merge_forward && slot_of == nullptr;
@@ -567,6 +630,7 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
             const KGen &g = (*gens)[size_t(i)];
             FGen fg{};
             h.slot = nslots++;
+            b.slot_ops.push_back(i);
             if (g.mat.empty()) {
                 h.gkind = kGenDiag;
                 fg.g[0] = cx<double>{g.g_even, 0.0};
@@ -787,11 +851,10 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
         Built b = build_pass(p, udag, n, true, &gens, &slot_col, rb, true, true);
         std::vector<int64_t> cols;
         std::vector<double> facs;
-        for (auto oi = p.ops.rbegin(); oi != p.ops.rend(); ++oi)
-            if (slot_col[size_t(*oi)] >= 0) {
-                cols.push_back(slot_col[size_t(*oi)]);
-                facs.push_back(factor[size_t(*oi)]);
-            }
+        for (int oi : b.slot_ops) {
+            cols.push_back(slot_col[size_t(oi)]);
+            facs.push_back(factor[size_t(oi)]);
+        }
         const size_t pb = (sizeof(PassDesc) + 255) & ~size_t(255);
         const size_t ob = (b.ops.size() * sizeof(FOp) + 255) & ~size_t(255);
         const size_t gb = (b.gens.size() * sizeof(FGen) + 255) & ~size_t(255);

@@ -810,6 +810,7 @@ struct Built {
     PassDesc pd;
     std::vector<FOp> ops;
     std::vector<FGen> gens; // one per slot
+    std::vector<int> slot_ops; // per slot: the index into kops of its gate
 };
 
 // reverse: gates in reverse order (the adjoint sweep); gens / slot_of mark
