@@ -649,7 +649,23 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
         }
         // bits this gate touches invalidate pending merges on them
         const uint64_t touched = k.kind == KOp::DIAG ? (k.cmask | k.pmask) : (k.cmask | (1ull << k.tbits[0]));
-        if (merge && single_bit >= 0 && last_single[size_t(single_bit)] >= 0) {
+        // With diagonal runs a diagonal gate costs less as a run member (on
+        // a register bit it is folded into the run's per-register constant)
+        // than as a factor that turns a real 2x2 complex, and two diagonal
+        // gates stay diagonal in a run: such pairs are not merged.
+        auto real_dense = [](const HOp &x) {
+            if (x.kind == kDiag) return false;
+            for (int q = 0; q < 4; ++q)
+                if (x.f.m[q].im != 0.0) return false;
+            return true;
+        };
+        auto keep_apart = [&](const HOp &x, const HOp &y) {
+            if (!diag_runs || (x.kind != kDiag && y.kind != kDiag)) return false;
+            const HOp &other = x.kind == kDiag ? y : x;
+            return other.kind == kDiag || real_dense(other);
+        };
+        if (merge && single_bit >= 0 && last_single[size_t(single_bit)] >= 0 &&
+            !keep_apart(h, hops[size_t(last_single[size_t(single_bit)])])) {
             // fold into the earlier gate on the same bit: M <- M_new * M_old
             HOp &prev = hops[size_t(last_single[size_t(single_bit)])];
             const cx<double> zero{0, 0};
