@@ -463,6 +463,50 @@ def test_fused_diagonal_runs_match_oracle(n, bits):
         N.set_tile_bits(4, 3)
 
 
+def controlled_fusable_circuit(n, rng, count):
+    """Only gates that fit fused passes: rotations, controlled single-qubit
+    gates and diagonal gates on overlapping wires.  Inside a pass, diagonal
+    gates are moved past dense gates that commute with them (controls are
+    used diagonally, targets are not), which these circuits exercise on every
+    wire role."""
+    one = ["RX", "RY", "RZ", "PhaseShift", "Hadamard", "S", "PauliX"]
+    ctrl = ["CNOT", "CY", "CRX", "CRY"]
+    diag = ["CZ", "CRZ", "ControlledPhaseShift", "IsingZZ"]
+    ops = []
+    for _ in range(count):
+        r = rng.random()
+        if r < 0.35:
+            g, w = one[rng.integers(len(one))], [int(rng.integers(n))]
+        elif r < 0.65:
+            g, w = ctrl[rng.integers(len(ctrl))], [int(x) for x in rng.choice(n, 2, replace=False)]
+        elif r < 0.95:
+            g, w = diag[rng.integers(len(diag))], [int(x) for x in rng.choice(n, 2, replace=False)]
+        else:
+            g, w = "MultiRZ", [int(x) for x in rng.choice(n, 3, replace=False)]
+        ops.append((g, w, list(rng.uniform(-3, 3, N.PARAM_COUNT.get(g, 0))), bool(rng.integers(2))))
+    return ops
+
+
+@pytest.mark.parametrize("bits", [(4, 3), (3, 3)])
+@pytest.mark.parametrize("n", [12, 13])
+def test_fused_reordered_passes_match_oracle(n, bits):
+    rng = np.random.default_rng(1700 + n)
+    ops = controlled_fusable_circuit(n, rng, 260)
+    init = rand_state(n, rng)
+    psi = po.apply_ops(n, init, ops)
+    bras = obs_states(n, psi, [[("PauliZ", [1])], [("PauliX", [n - 2]), ("PauliZ", [4])]])
+    npar = sum(1 for o in ops if o[2])
+    tp = list(range(npar))
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    try:
+        N.set_tile_bits(*bits)
+        close128(N.State(n).upload(init).apply(ops).download(), psi)
+        got = N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp)
+        close128(got, ref)
+    finally:
+        N.set_tile_bits(4, 3)
+
+
 def test_fused_adjoint_config2_shape():
     # RY/RZ + CZ brick (the 28-qubit benchmark circuit family) at 14 qubits
     from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
