@@ -501,8 +501,12 @@ def test_fused_reordered_passes_match_oracle(n, bits):
     try:
         N.set_tile_bits(*bits)
         close128(N.State(n).upload(init).apply(ops).download(), psi)
+        close64(N.State(n, np.complex64).upload(init.astype(np.complex64)).apply(ops).download(), psi)
         got = N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp)
         close128(got, ref)
+        got64 = N.adjoint_jacobian(N.State(n, np.complex64).upload(psi.astype(np.complex64)),
+                                   [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
+        close64(got64, ref)
     finally:
         N.set_tile_bits(4, 3)
 
