@@ -511,6 +511,35 @@ def test_fused_reordered_passes_match_oracle(n, bits):
         N.set_tile_bits(4, 3)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_fused_random_sweep_vs_oracle(seed):
+    """Seeded random circuits of 14-17 qubits: mostly fusable gates (so the
+    passes are long and their gates are regrouped), with a few gates that
+    run on their own and split the passes.  Forward state and adjoint
+    Jacobian against the oracle, complex128 and complex64."""
+    rng = np.random.default_rng(4200 + seed)
+    n = 14 + seed % 4
+    ops = controlled_fusable_circuit(n, rng, 180 + 20 * seed)
+    split = ["SWAP", "IsingXX", "IsingYY", "SingleExcitation"]
+    for _ in range(6):
+        g = split[rng.integers(len(split))]
+        w = [int(x) for x in rng.choice(n, 2, replace=False)]
+        ops.insert(int(rng.integers(len(ops))), (g, w, list(rng.uniform(-3, 3, N.PARAM_COUNT.get(g, 0))), False))
+    init = rand_state(n, rng)
+    psi = po.apply_ops(n, init, ops)
+    close128(N.State(n).upload(init).apply(ops).download(), psi)
+    close64(N.State(n, np.complex64).upload(init.astype(np.complex64)).apply(ops).download(), psi)
+    bras = obs_states(n, psi, [[("PauliZ", [0]), ("PauliZ", [n - 1])], [("PauliY", [n // 2])]])
+    npar = sum(1 for o in ops if o[2])
+    tp = sorted(int(x) for x in rng.choice(npar, (2 * npar) // 3, replace=False))
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    got = N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp)
+    close128(got, ref)
+    got64 = N.adjoint_jacobian(N.State(n, np.complex64).upload(psi.astype(np.complex64)),
+                               [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
+    close64(got64, ref)
+
+
 def test_fused_adjoint_config2_shape():
     # RY/RZ + CZ brick (the 28-qubit benchmark circuit family) at 14 qubits
     from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
