@@ -131,6 +131,16 @@ lgh_status lgh_set_fusion(int enabled);
  * of the adjoint passes; a wavefront holds 2^(6 + bits) amplitudes per state.
  * Defaults 4 and 3.  No counterpart in the reference; a tuning knob. */
 lgh_status lgh_set_tile_bits(int forward, int adjoint);
+/* Plan of the fused passes that `ops` get on `num_qubits` qubits, computed on
+ * the host without touching a device: passes in total, of those the
+ * standalone ones (gates that fit no fused pass), and over the fused passes
+ * the diagonal runs and the diagonal gates applied as their members.
+ * adjoint 0: the forward passes of lgh_apply; 1: the reverse passes of
+ * lgh_adjoint_jacobian with every parametric gate trainable.  State
+ * preparation ops (gate < 0) are skipped.  No counterpart in the reference,
+ * which applies one kernel per gate; for inspecting fusion and for tests. */
+lgh_status lgh_fusion_plan(int32_t num_qubits, const lgh_op *ops, int64_t num_ops, int32_t adjoint, lgh_dtype dtype,
+                           int32_t *passes, int32_t *standalone, int32_t *runs, int32_t *run_members);
 /* Launch timing of the hot kernels with HIP events recorded on their
  * launch stream (off by default; no counterpart in the reference).  Enabling
  * or disabling clears what was recorded.  kind 0: fused forward passes,

@@ -834,8 +834,47 @@ void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops) {
         run_forward<float>(sv, kops, passes, rb);
 }
 
-int count_passes(const std::vector<KOp> &kops, int n) {
-    return static_cast<int>(schedule_passes(kops, n, {}, g_rb_fwd.load(), fixed_lanes(LGH_C128)).size());
+FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, lgh_dtype dtype) {
+    std::vector<KOp> kops;
+    std::vector<KGen> gens;
+    std::vector<int> slot;
+    int col = 0;
+    for (int64_t i = 0; i < num_ops; ++i) {
+        const lgh_op &o = ops[i];
+        if (o.gate < 0) continue; // state preparation
+        kops.push_back(compile_op(o, n, adjoint));
+        if (adjoint && o.num_params > 0) {
+            gens.push_back(compile_generator(o, n, kops.back()));
+            slot.push_back(col++);
+        } else {
+            gens.emplace_back();
+            slot.push_back(-1);
+        }
+    }
+    const int rb = adjoint ? g_rb_adj.load() : g_rb_fwd.load();
+    const std::vector<Pass> passes = schedule_passes(kops, n, adjoint ? slot : std::vector<int>{}, rb, fixed_lanes(dtype));
+    FusionPlan plan{};
+    for (const Pass &p : passes) {
+        ++plan.passes;
+        if (p.standalone) {
+            ++plan.standalone;
+            continue;
+        }
+        const Built b = adjoint ? build_pass(p, kops, n, true, &gens, &slot, rb, true, true)
+                                : build_pass(p, kops, n, false, nullptr, nullptr, rb, true, true);
+        for (size_t j = 0; j < b.ops.size();) {
+            const FOp &f = b.ops[j];
+            if ((f.meta & 15u) != kDiagRun) {
+                ++j;
+                continue;
+            }
+            const int nmem = int(f.pad & 0xffffu), ndata = int((f.pad >> 16) & 0xffu);
+            ++plan.runs;
+            plan.run_members += nmem;
+            j += size_t(1 + ndata + nmem);
+        }
+    }
+    return plan;
 }
 
 template <class R>
@@ -926,4 +965,26 @@ extern "C" lgh_status lgh_set_fusion(int enabled) {
 
 extern "C" lgh_status lgh_set_tile_bits(int forward, int adjoint) {
     return guard([&] { set_tile_bits(forward, adjoint); });
+}
+
+extern "C" lgh_status lgh_fusion_plan(int32_t num_qubits, const lgh_op *ops, int64_t num_ops, int32_t adjoint,
+                                      lgh_dtype dtype, int32_t *passes, int32_t *standalone, int32_t *runs,
+                                      int32_t *run_members) {
+    return guard([&] {
+        LGH_REQUIRE(num_qubits >= 1 && num_qubits <= 63, "bad number of qubits");
+        LGH_REQUIRE(num_ops >= 0 && (num_ops == 0 || ops), "bad op list");
+        LGH_REQUIRE(dtype == LGH_C64 || dtype == LGH_C128, "bad dtype");
+        LGH_REQUIRE(passes && standalone && runs && run_members, "null output");
+        for (int64_t i = 0; i < num_ops; ++i) {
+            if (ops[i].gate < 0) continue;
+            LGH_REQUIRE(!adjoint || ops[i].num_params <= 1,
+                        "The operation is not supported using the adjoint differentiation method");
+            validate_op(ops[i], num_qubits);
+        }
+        const FusionPlan plan = plan_passes(ops, num_ops, num_qubits, adjoint != 0, dtype);
+        *passes = plan.passes;
+        *standalone = plan.standalone;
+        *runs = plan.runs;
+        *run_members = plan.run_members;
+    });
 }

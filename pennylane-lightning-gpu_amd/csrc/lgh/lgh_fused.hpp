@@ -36,7 +36,12 @@ bool fusable(const KOp &k, int n, int rb);
 // fixed).
 std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot,
                                   int rb, int fixed_lanes = 6);
-int count_passes(const std::vector<KOp> &kops, int n);
+// Host-side plan of the fused passes a gate sequence gets (no device use):
+// forward passes, or adjoint passes with every parametric gate trainable.
+struct FusionPlan {
+    int32_t passes = 0, standalone = 0, runs = 0, run_members = 0;
+};
+FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, lgh_dtype dtype);
 
 // Forward application of a compiled op list through fused passes.
 void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops);

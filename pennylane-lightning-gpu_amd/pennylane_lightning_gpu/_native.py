@@ -111,6 +111,7 @@ def lib():
         L.lgh_stream_destroy.argtypes = [vp]
         L.lgh_stream_synchronize.argtypes = [vp]
         L.lgh_set_tile_bits.argtypes = [C.c_int, C.c_int]
+        L.lgh_fusion_plan.argtypes = [C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int] + [C.POINTER(C.c_int32)] * 4
         L.lgh_profile_enable.argtypes = [C.c_int]
         L.lgh_profile_read.argtypes = [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.lgh_tmpl_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Op), C.c_int32, vp, vp, vp, C.c_int32,
@@ -142,6 +143,17 @@ def set_fusion(enabled: bool):
 def set_tile_bits(forward: int, adjoint: int):
     """Register bits (3 or 4) of the fused forward and adjoint passes."""
     check(lib().lgh_set_tile_bits(int(forward), int(adjoint)))
+
+
+def fusion_plan(n, ops, adjoint=False, dtype=np.complex128):
+    """Fused-pass plan of ``ops`` on ``n`` qubits, computed on the host (no
+    device): dict with ``passes``, ``standalone``, ``runs`` (diagonal runs)
+    and ``run_members`` (diagonal gates applied inside runs)."""
+    arr, keep = make_ops(ops) if ops else (None, None)
+    out = [C.c_int32() for _ in range(4)]
+    code = LGH_C128 if np.dtype(dtype) == np.complex128 else LGH_C64
+    check(lib().lgh_fusion_plan(int(n), arr, len(ops), int(bool(adjoint)), code, *[C.byref(o) for o in out]))
+    return dict(zip(("passes", "standalone", "runs", "run_members"), (o.value for o in out)))
 
 
 def device_count() -> int:

@@ -1,0 +1,70 @@
+"""Host-side plan of the fused passes (lgh_fusion_plan): no device needed.
+
+The plan reports how many passes a gate sequence gets, how many of them are
+standalone, and how many diagonal gates are applied inside diagonal runs.
+These tests pin the scheduler's grouping of diagonal gates: inside a pass a
+diagonal gate may move past a gate that uses its qubits only diagonally (a
+control, a phase), never past a dense gate on one of its qubits.
+"""
+import pytest
+
+N = pytest.importorskip("pennylane_lightning_gpu._native")
+
+
+def plan(n, ops, adjoint=False):
+    return N.fusion_plan(n, ops, adjoint=adjoint)
+
+
+def test_benchmark_circuit_plan():
+    from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
+
+    ops = ry_rz_cz_brick(28, 6, seed=1337)  # 498 gates: 168 RY, 168 RZ, 162 CZ
+    ndiag = sum(1 for o in ops if o[0] in ("RZ", "CZ"))
+    adj = plan(28, ops, adjoint=True)
+    assert adj["standalone"] == 0 and adj["passes"] <= 15
+    # every diagonal gate of the reverse sweep is a member of a diagonal run
+    assert adj["run_members"] == ndiag
+    fwd = plan(28, ops)
+    assert fwd["standalone"] == 0 and fwd["passes"] <= 8
+    assert fwd["run_members"] >= ndiag - 2
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_commuting_diagonal_gates_form_one_run(adjoint):
+    # RZ / CZ on wires that the RY gates do not touch: one run of four
+    ops = [("RZ", [0], [0.1], False), ("RY", [1], [0.2], False), ("RZ", [2], [0.3], False),
+           ("RY", [3], [0.4], False), ("RZ", [4], [0.5], False), ("RY", [5], [0.6], False),
+           ("CZ", [6, 7], [], False), ("RY", [8], [0.7], False)]
+    assert plan(10, ops, adjoint) == {"passes": 1, "standalone": 0, "runs": 1, "run_members": 4}
+
+
+def test_diagonal_gates_stay_behind_dense_gates_on_their_wire():
+    ops = [("RZ", [0], [0.1], False), ("RY", [0], [0.2], False), ("RZ", [0], [0.3], False), ("RY", [0], [0.4], False)]
+    assert plan(10, ops, adjoint=True)["runs"] == 0
+
+
+@pytest.mark.parametrize("adjoint", [False, True])
+def test_controls_commute_with_diagonal_gates_targets_do_not(adjoint):
+    # RZ on the control of a CNOT moves past it: the two RZ form a run
+    on_control = [("RZ", [0], [0.1], False), ("CNOT", [0, 1], [], False), ("RZ", [0], [0.3], False)]
+    assert plan(10, on_control, adjoint)["run_members"] == 2
+    # RZ on its target does not
+    on_target = [("RZ", [1], [0.1], False), ("CNOT", [0, 1], [], False), ("RZ", [1], [0.3], False)]
+    assert plan(10, on_target, adjoint)["run_members"] == 0
+
+
+def test_standalone_gates_and_small_states():
+    ops = [("RY", [0], [0.1], False), ("SWAP", [0, 5], [], False), ("RY", [5], [0.2], False)]
+    assert plan(12, ops) == {"passes": 3, "standalone": 1, "runs": 0, "run_members": 0}
+    # below 6 lane + 4 register bits no gate fits a fused forward pass
+    small = [("RY", [0], [0.1], False), ("RZ", [1], [0.2], False)]
+    assert plan(9, small)["standalone"] == 2
+
+
+def test_plan_errors():
+    with pytest.raises(N.LghError, match="adjoint"):
+        plan(10, [("Rot", [0], [0.1, 0.2, 0.3], False)], adjoint=True)
+    with pytest.raises(N.LghError):
+        plan(0, [("RY", [0], [0.1], False)])
+    with pytest.raises(N.LghError):
+        plan(4, [("RY", [7], [0.1], False)])
