@@ -242,7 +242,12 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
         std::stable_sort(chosen.begin(), chosen.end(), [&](int x, int y) { return weight[size_t(x)] < weight[size_t(y)]; });
         std::vector<int> lanes(chosen.begin(), chosen.begin() + (kLaneBits - fixed_lanes));
         std::vector<int> regs(chosen.begin() + (kLaneBits - fixed_lanes), chosen.end());
-        std::sort(lanes.begin(), lanes.end());
+        // free lane positions in order of cost: the ones inside a 16-lane
+        // row exchange with one DPP move per dword, positions 4 and 5 swap
+        // rows; the lane bit with the most dense gates takes the cheapest
+        std::stable_sort(lanes.begin(), lanes.end(), [&](int x, int y) {
+            return weight[size_t(x)] != weight[size_t(y)] ? weight[size_t(x)] > weight[size_t(y)] : x < y;
+        });
         std::sort(regs.begin(), regs.end());
         for (int q = 0; q < kLaneBits; ++q) p.lane_bits[q] = q < fixed_lanes ? q : lanes[size_t(q - fixed_lanes)];
         for (int r = 0; r < rb; ++r) p.reg_bits[r] = regs[size_t(r)];
