@@ -446,6 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_slot_store(const double *__restrict_
 // host form of FOp before its fields are packed into meta
 struct HOp {
     uint32_t kind = 0, gkind = 0, dneg = 0;
+    uint32_t rot = 0; // matrix [[1, -t], [t, 1]] (a rotation with its cosine factored out)
     int loc = 0, slot = -1;
     uint64_t cm_reg = 0, pm_reg = 0; // control / parity bits among the register bits
     FOp f{};
@@ -459,7 +460,7 @@ FOp pack(const HOp &h, const uint64_t (&reg_off)[kMaxNR], int nr) {
     const bool isx = h.kind != kDiag && f.m[0].re == 0.0 && f.m[0].im == 0.0 && f.m[1].re == 1.0 &&
                      f.m[1].im == 0.0 && f.m[2].re == 1.0 && f.m[2].im == 0.0 && f.m[3].re == 0.0 && f.m[3].im == 0.0;
     f.meta = h.kind | uint32_t(h.loc) << 4 | uint32_t(h.slot + 1) << 8 | h.gkind << 16 | h.dneg << 20 |
-             uint32_t(real) << 21 | uint32_t(dsym) << 22 | uint32_t(isx) << 23;
+             uint32_t(real) << 21 | uint32_t(dsym) << 22 | uint32_t(isx) << 23 | h.rot << 28;
     uint32_t rok = 0, rpar = 0;
     for (int k = 0; k < kMaxNR; ++k) {
         // registers beyond nr do not exist; their bits read as "controls hold"
@@ -706,6 +707,41 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
         hops.push_back(h);
         if (merge && single_bit >= 0) last_single[size_t(single_bit)] = int(hops.size()) - 1;
     }
+    // Rotations [[c, -s], [s, c]] with |c| >= |s| (RY, its adjoint, products
+    // of them) become c [[1, -t], [t, 1]], t = s / c: one FMA per output
+    // component instead of a multiply and an FMA.  The factors c multiply
+    // into one scale C for the whole pass, applied by a diagonal gate at its
+    // end.  Until then the registers hold the states divided by the running
+    // product, so the overlap of a gate taken there is divided by its square:
+    // slot_scale carries that square into the Jacobian factor.
+    b.slot_scale.assign(size_t(nslots), 1.0);
+    size_t scale_hop = hops.size() + 1; // index of the closing scale gate, if any
+    if (diag_runs) {
+        double scale = 1.0;
+        for (HOp &h : hops) {
+            if (h.slot >= 0) b.slot_scale[size_t(h.slot)] = scale * scale;
+            if (h.kind == kDiag || h.cm_reg != 0 || h.f.cm_rest != 0) continue;
+            const cx<double> *m = h.f.m;
+            if (m[0].im != 0.0 || m[1].im != 0.0 || m[2].im != 0.0 || m[3].im != 0.0) continue;
+            const double c = m[0].re;
+            if (m[3].re != c || m[2].re != -m[1].re || c == 0.0 || std::abs(c) < std::abs(m[2].re)) continue;
+            const double t = m[2].re / c;
+            h.f.m[0] = cx<double>{1.0, 0.0};
+            h.f.m[1] = cx<double>{-t, 0.0};
+            h.f.m[2] = cx<double>{t, 0.0};
+            h.f.m[3] = cx<double>{1.0, 0.0};
+            h.rot = 1;
+            scale *= c;
+        }
+        if (scale != 1.0) {
+            HOp fin;
+            fin.kind = kDiag;
+            fin.loc = kDiagNoReg;
+            fin.f.m[0] = fin.f.m[1] = cx<double>{scale, 0.0};
+            scale_hop = hops.size();
+            hops.push_back(fin);
+        }
+    }
     for (size_t i = 0; i < hops.size();) {
         // a diagonal member needs a diagonal generator when it is trainable
         auto member = [&](const HOp &h) { return h.kind == kDiag && (h.slot < 0 || h.gkind == kGenDiag); };
@@ -723,6 +759,7 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
             continue;
         }
         // diagonal run over hops[i, j): header, P[k] data ops, members
+        if (scale_hop >= i && scale_hop < j) b.scale_members = 1;
         std::vector<std::complex<double>> P(size_t(nr), 1.0);
         uint32_t flags = 0;
         std::vector<FOp> members;
@@ -878,6 +915,7 @@ FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, 
             plan.run_members += nmem;
             j += size_t(1 + ndata + nmem);
         }
+        plan.run_members -= b.scale_members; // the closing scale gate is no gate of the list
     }
     return plan;
 }
@@ -911,9 +949,10 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
         Built b = build_pass(p, udag, n, true, &gens, &slot_col, rb, true, true);
         std::vector<int64_t> cols;
         std::vector<double> facs;
-        for (int oi : b.slot_ops) {
+        for (size_t s = 0; s < b.slot_ops.size(); ++s) {
+            const int oi = b.slot_ops[s];
             cols.push_back(slot_col[size_t(oi)]);
-            facs.push_back(factor[size_t(oi)]);
+            facs.push_back(factor[size_t(oi)] * b.slot_scale[s]);
         }
         const size_t pb = (sizeof(PassDesc) + 255) & ~size_t(255);
         const size_t ob = (b.ops.size() * sizeof(FOp) + 255) & ~size_t(255);

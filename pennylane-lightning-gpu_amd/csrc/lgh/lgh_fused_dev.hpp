@@ -56,6 +56,9 @@ __device__ __forceinline__ bool f_real(const FOp &o) { return (o.meta >> 21) & 1
 __device__ __forceinline__ bool f_dsym(const FOp &o) { return (o.meta >> 22) & 1u; }
 // dense 2x2 equal to [[0, 1], [1, 0]] (X, CNOT, Toffoli)
 __device__ __forceinline__ bool f_isx(const FOp &o) { return (o.meta >> 23) & 1u; }
+// dense 2x2 equal to [[1, -t], [t, 1]]: a rotation whose cosine the host
+// moved into a scale applied at the end of the pass
+__device__ __forceinline__ bool f_rot(const FOp &o) { return (o.meta >> 28) & 1u; }
 // every register satisfies the register-bit controls (the controls, if any,
 // are on lane or higher bits)
 __device__ __forceinline__ bool f_regs_ok(const FOp &o) { return (o.rbits & 0xffffu) == 0xffffu; }
@@ -174,9 +177,13 @@ template <class R> struct M2 {
 template <class R> __device__ __forceinline__ M2<R> m2(const FOp &op) {
     return M2<R>{lo<R>(op.m[0]), lo<R>(op.m[1]), lo<R>(op.m[2]), lo<R>(op.m[3])};
 }
-template <class R, bool Real> __device__ __forceinline__ void mat2(const M2<R> &m, cx<R> &x0, cx<R> &x1) {
+template <class R, bool Real, bool Rot = false>
+__device__ __forceinline__ void mat2(const M2<R> &m, cx<R> &x0, cx<R> &x1) {
     cx<R> y0, y1;
-    if constexpr (Real) {
+    if constexpr (Rot) {
+        y0 = rfma(m.m01.re, x1, x0);
+        y1 = rfma(m.m10.re, x0, x1);
+    } else if constexpr (Real) {
         y0 = rfma(m.m01.re, x1, rmul(m.m00.re, x0));
         y1 = rfma(m.m11.re, x1, rmul(m.m10.re, x0));
     } else {
@@ -187,33 +194,37 @@ template <class R, bool Real> __device__ __forceinline__ void mat2(const M2<R> &
     x1 = y1;
 }
 // lane bit L <= 3: each lane updates its own amplitude from the partner's
-template <class R, bool Real> __device__ __forceinline__ cx<R> row2(cx<R> cs, cx<R> cp, cx<R> own, cx<R> partner) {
-    if constexpr (Real)
+template <class R, bool Real, bool Rot = false>
+__device__ __forceinline__ cx<R> row2(cx<R> cs, cx<R> cp, cx<R> own, cx<R> partner) {
+    if constexpr (Rot)
+        return rfma(cp.re, partner, own);
+    else if constexpr (Real)
         return rfma(cp.re, partner, rmul(cs.re, own));
     else
         return cfma(cp, partner, cmul(cs, own));
 }
 
 // --- uncontrolled gates -------------------------------------------------
-template <class R, int NR, int B, bool Real> __device__ __forceinline__ void reg_plain(const M2<R> &m, cx<R> (&v)[NR]) {
+template <class R, int NR, int B, bool Real, bool Rot = false>
+__device__ __forceinline__ void reg_plain(const M2<R> &m, cx<R> (&v)[NR]) {
 #pragma unroll
     for (int k = 0; k < NR; ++k)
-        if (!(k & (1 << B))) mat2<R, Real>(m, v[k], v[k | (1 << B)]);
+        if (!(k & (1 << B))) mat2<R, Real, Rot>(m, v[k], v[k | (1 << B)]);
 }
-template <class R, int NR, int L, bool Real>
+template <class R, int NR, int L, bool Real, bool Rot = false>
 __device__ __forceinline__ void lane_plain(const M2<R> &m, int lane, cx<R> (&v)[NR]) {
     if constexpr (L >= 4) {
 #pragma unroll
         for (int k = 0; k < NR; k += 2) {
             row_swap_cx<L>(v[k], v[k + 1]);
-            mat2<R, Real>(m, v[k], v[k + 1]);
+            mat2<R, Real, Rot>(m, v[k], v[k + 1]);
             row_swap_cx<L>(v[k], v[k + 1]);
         }
     } else {
         const bool hi = (lane >> L) & 1;
         const cx<R> cs = hi ? m.m11 : m.m00, cp = hi ? m.m10 : m.m01;
 #pragma unroll
-        for (int k = 0; k < NR; ++k) v[k] = row2<R, Real>(cs, cp, v[k], xor_lane_cx<L>(v[k], lane));
+        for (int k = 0; k < NR; ++k) v[k] = row2<R, Real, Rot>(cs, cp, v[k], xor_lane_cx<L>(v[k], lane));
     }
 }
 
@@ -346,6 +357,8 @@ template <class R, int NR, int B> __device__ __forceinline__ void reg_gate(const
                 reg_masked<R, NR, B, false, false>(op, m, lane_ok, v);
         }
     }
+    else if (f_rot(op))
+        reg_plain<R, NR, B, true, true>(m, v);
     else if (f_real(op))
         reg_plain<R, NR, B, true>(m, v);
     else
@@ -367,6 +380,8 @@ __device__ __forceinline__ void lane_gate(const FOp &op, int lane, bool lane_ok,
                 lane_masked<R, NR, L, false, false>(op, m, lane, lane_ok, v);
         }
     }
+    else if (f_rot(op))
+        lane_plain<R, NR, L, true, true>(m, lane, v);
     else if (f_real(op))
         lane_plain<R, NR, L, true>(m, lane, v);
     else
@@ -484,7 +499,7 @@ __device__ __forceinline__ double lane_dot_masked(const FOp &op, const FGen &g, 
 // One reverse step of an uncontrolled dense gate on (lambda, bra): the
 // generator overlap (with Gen) and U^dagger on both states, sharing the lane
 // exchanges between them.
-template <class R, int NR, int B, bool Real, bool Gen>
+template <class R, int NR, int B, bool Real, bool Gen, bool Rot = false>
 __device__ __forceinline__ double adj_reg_plain(const M2<R> &m, uint32_t gk, const FGen *g, cx<R> (&a)[NR],
                                                 cx<R> (&b)[NR]) {
     double s = 0.0;
@@ -493,12 +508,12 @@ __device__ __forceinline__ double adj_reg_plain(const M2<R> &m, uint32_t gk, con
         if (k & (1 << B)) continue;
         const int k1 = k | (1 << B);
         if constexpr (Gen) s += pair_dot<R>(gk, *g, a[k], a[k1], b[k], b[k1]);
-        mat2<R, Real>(m, a[k], a[k1]);
-        mat2<R, Real>(m, b[k], b[k1]);
+        mat2<R, Real, Rot>(m, a[k], a[k1]);
+        mat2<R, Real, Rot>(m, b[k], b[k1]);
     }
     return s;
 }
-template <class R, int NR, int L, bool Real, bool Gen>
+template <class R, int NR, int L, bool Real, bool Gen, bool Rot = false>
 __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, const FGen *g, int lane,
                                                  cx<R> (&a)[NR], cx<R> (&b)[NR]) {
     double s = 0.0;
@@ -508,8 +523,8 @@ __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, co
             row_swap_cx<L>(a[k], a[k + 1]);
             row_swap_cx<L>(b[k], b[k + 1]);
             if constexpr (Gen) s += pair_dot<R>(gk, *g, a[k], a[k + 1], b[k], b[k + 1]);
-            mat2<R, Real>(m, a[k], a[k + 1]);
-            mat2<R, Real>(m, b[k], b[k + 1]);
+            mat2<R, Real, Rot>(m, a[k], a[k + 1]);
+            mat2<R, Real, Rot>(m, b[k], b[k + 1]);
             row_swap_cx<L>(a[k], a[k + 1]);
             row_swap_cx<L>(b[k], b[k + 1]);
         }
@@ -520,11 +535,27 @@ __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, co
         for (int k = 0; k < NR; ++k) {
             const cx<R> pa = xor_lane_cx<L>(a[k], lane);
             if constexpr (Gen) s += row_dot<R>(gk, *g, hi, a[k], pa, b[k]);
-            a[k] = row2<R, Real>(cs, cp, a[k], pa);
-            b[k] = row2<R, Real>(cs, cp, b[k], xor_lane_cx<L>(b[k], lane));
+            a[k] = row2<R, Real, Rot>(cs, cp, a[k], pa);
+            b[k] = row2<R, Real, Rot>(cs, cp, b[k], xor_lane_cx<L>(b[k], lane));
         }
     }
     return s;
+}
+
+// the three matrix forms of an uncontrolled dense step
+template <class R, int NR, int B, bool Gen>
+__device__ __forceinline__ double adj_reg_any(const M2<R> &m, bool rot, bool real, uint32_t gk, const FGen *g,
+                                              cx<R> (&a)[NR], cx<R> (&b)[NR]) {
+    if (rot) return adj_reg_plain<R, NR, B, true, Gen, true>(m, gk, g, a, b);
+    if (real) return adj_reg_plain<R, NR, B, true, Gen>(m, gk, g, a, b);
+    return adj_reg_plain<R, NR, B, false, Gen>(m, gk, g, a, b);
+}
+template <class R, int NR, int L, bool Gen>
+__device__ __forceinline__ double adj_lane_any(const M2<R> &m, bool rot, bool real, uint32_t gk, const FGen *g,
+                                               int lane, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
+    if (rot) return adj_lane_plain<R, NR, L, true, Gen, true>(m, gk, g, lane, a, b);
+    if (real) return adj_lane_plain<R, NR, L, true, Gen>(m, gk, g, lane, a, b);
+    return adj_lane_plain<R, NR, L, false, Gen>(m, gk, g, lane, a, b);
 }
 
 template <class R, int NR, bool Gen>
@@ -532,26 +563,25 @@ __device__ __forceinline__ double adj_plain_dense(const FOp &op, const FGen *g, 
                                                   cx<R> (&b)[NR]) {
     const M2<R> m = m2<R>(op);
     const uint32_t gk = f_gkind(op);
-    const bool real = f_real(op);
+    const bool real = f_real(op), rot = f_rot(op);
     const int loc = f_loc(op);
     if (f_kind(op) == kRegDense) {
         switch (loc) {
-        case 0: return real ? adj_reg_plain<R, NR, 0, true, Gen>(m, gk, g, a, b) : adj_reg_plain<R, NR, 0, false, Gen>(m, gk, g, a, b);
-        case 1: return real ? adj_reg_plain<R, NR, 1, true, Gen>(m, gk, g, a, b) : adj_reg_plain<R, NR, 1, false, Gen>(m, gk, g, a, b);
-        case 2: return real ? adj_reg_plain<R, NR, 2, true, Gen>(m, gk, g, a, b) : adj_reg_plain<R, NR, 2, false, Gen>(m, gk, g, a, b);
+        case 0: return adj_reg_any<R, NR, 0, Gen>(m, rot, real, gk, g, a, b);
+        case 1: return adj_reg_any<R, NR, 1, Gen>(m, rot, real, gk, g, a, b);
+        case 2: return adj_reg_any<R, NR, 2, Gen>(m, rot, real, gk, g, a, b);
         default:
-            if constexpr (NR > 8)
-                return real ? adj_reg_plain<R, NR, 3, true, Gen>(m, gk, g, a, b) : adj_reg_plain<R, NR, 3, false, Gen>(m, gk, g, a, b);
+            if constexpr (NR > 8) return adj_reg_any<R, NR, 3, Gen>(m, rot, real, gk, g, a, b);
             return 0.0;
         }
     }
     switch (loc) {
-    case 0: return real ? adj_lane_plain<R, NR, 0, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 0, false, Gen>(m, gk, g, lane, a, b);
-    case 1: return real ? adj_lane_plain<R, NR, 1, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 1, false, Gen>(m, gk, g, lane, a, b);
-    case 2: return real ? adj_lane_plain<R, NR, 2, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 2, false, Gen>(m, gk, g, lane, a, b);
-    case 3: return real ? adj_lane_plain<R, NR, 3, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 3, false, Gen>(m, gk, g, lane, a, b);
-    case 4: return real ? adj_lane_plain<R, NR, 4, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 4, false, Gen>(m, gk, g, lane, a, b);
-    default: return real ? adj_lane_plain<R, NR, 5, true, Gen>(m, gk, g, lane, a, b) : adj_lane_plain<R, NR, 5, false, Gen>(m, gk, g, lane, a, b);
+    case 0: return adj_lane_any<R, NR, 0, Gen>(m, rot, real, gk, g, lane, a, b);
+    case 1: return adj_lane_any<R, NR, 1, Gen>(m, rot, real, gk, g, lane, a, b);
+    case 2: return adj_lane_any<R, NR, 2, Gen>(m, rot, real, gk, g, lane, a, b);
+    case 3: return adj_lane_any<R, NR, 3, Gen>(m, rot, real, gk, g, lane, a, b);
+    case 4: return adj_lane_any<R, NR, 4, Gen>(m, rot, real, gk, g, lane, a, b);
+    default: return adj_lane_any<R, NR, 5, Gen>(m, rot, real, gk, g, lane, a, b);
     }
 }
 
@@ -811,6 +841,8 @@ struct Built {
     std::vector<FOp> ops;
     std::vector<FGen> gens; // one per slot
     std::vector<int> slot_ops; // per slot: the index into kops of its gate
+    std::vector<double> slot_scale; // per slot: factor of its Jacobian entry (deferred rotation scales)
+    int scale_members = 0;          // 1 when the pass's closing scale gate is a diagonal-run member
 };
 
 // reverse: gates in reverse order (the adjoint sweep); gens / slot_of mark

@@ -39,8 +39,11 @@ def test_commuting_diagonal_gates_form_one_run(adjoint):
 
 
 def test_diagonal_gates_stay_behind_dense_gates_on_their_wire():
+    # the two RZ never share a run (one of them may close the pass in a run
+    # with the pass's own scale gate, which the plan does not count)
     ops = [("RZ", [0], [0.1], False), ("RY", [0], [0.2], False), ("RZ", [0], [0.3], False), ("RY", [0], [0.4], False)]
-    assert plan(10, ops, adjoint=True)["runs"] == 0
+    assert plan(10, ops, adjoint=True)["run_members"] <= 1
+    assert plan(10, ops)["run_members"] <= 1
 
 
 @pytest.mark.parametrize("adjoint", [False, True])
