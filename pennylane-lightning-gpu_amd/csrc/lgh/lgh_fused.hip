@@ -715,11 +715,29 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
     // product, so the overlap of a gate taken there is divided by its square:
     // slot_scale carries that square into the Jacobian factor.
     b.slot_scale.assign(size_t(nslots), 1.0);
-    size_t scale_hop = hops.size() + 1; // index of the closing scale gate, if any
+    // a diagonal member needs a diagonal generator when it is trainable
+    auto member = [&](const HOp &h) { return h.kind == kDiag && (h.slot < 0 || h.gkind == kGenDiag); };
+    // end of the diagonal run that starts at hops[i], or i when hops[i] is
+    // not the start of one (a run needs two members applied by the run)
+    auto run_end = [&](size_t i) {
+        size_t j = i;
+        while (diag_runs && j < hops.size() && member(hops[j])) ++j;
+        int deferred = 0;
+        for (size_t t = i; t < j; ++t) {
+            const HOp &h = hops[t];
+            const bool reg = (h.cm_reg | h.pm_reg) != 0, rest = (h.f.cm_rest | h.f.pm_rest) != 0;
+            deferred += !reg || !rest || h.dneg;
+        }
+        return deferred < 2 ? i : j;
+    };
+    size_t scale_hop = hops.size() + 1;           // index of a closing scale gate, if any
+    size_t fold_i = hops.size() + 1, fold_j = 0;  // the run that applies the scale, if any
+    double scale = 1.0;
     if (diag_runs) {
-        double scale = 1.0;
-        for (HOp &h : hops) {
-            if (h.slot >= 0) b.slot_scale[size_t(h.slot)] = scale * scale;
+        std::vector<double> before(hops.size(), 1.0); // running scale before each gate
+        for (size_t x = 0; x < hops.size(); ++x) {
+            HOp &h = hops[x];
+            before[x] = scale;
             if (h.kind == kDiag || h.cm_reg != 0 || h.f.cm_rest != 0) continue;
             const cx<double> *m = h.f.m;
             if (m[0].im != 0.0 || m[1].im != 0.0 || m[2].im != 0.0 || m[3].im != 0.0) continue;
@@ -734,34 +752,47 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
             scale *= c;
         }
         if (scale != 1.0) {
-            HOp fin;
-            fin.kind = kDiag;
-            fin.loc = kDiagNoReg;
-            fin.f.m[0] = fin.f.m[1] = cx<double>{scale, 0.0};
-            scale_hop = hops.size();
-            hops.push_back(fin);
+            // The scale is a global factor, so any point of the pass may
+            // apply it: the last diagonal run takes it into its per-register
+            // constants at no extra cost.  Gates after that run see states
+            // multiplied by the whole scale and divided by their running one.
+            for (size_t i = 0; i < hops.size();) {
+                const size_t j = run_end(i);
+                if (j == i) {
+                    ++i;
+                    continue;
+                }
+                fold_i = i;
+                fold_j = j;
+                i = j;
+            }
+            if (fold_j == 0) {
+                HOp fin; // no run: a diagonal gate closes the pass
+                fin.kind = kDiag;
+                fin.loc = kDiagNoReg;
+                fin.f.m[0] = fin.f.m[1] = cx<double>{scale, 0.0};
+                scale_hop = hops.size();
+                hops.push_back(fin);
+                before.push_back(scale);
+            }
+        }
+        for (size_t x = 0; x < before.size() && x < hops.size(); ++x) {
+            if (hops[x].slot < 0) continue;
+            const double r = x < fold_j || fold_j == 0 ? before[x] : before[x] / scale;
+            b.slot_scale[size_t(hops[x].slot)] = r * r;
         }
     }
     for (size_t i = 0; i < hops.size();) {
-        // a diagonal member needs a diagonal generator when it is trainable
-        auto member = [&](const HOp &h) { return h.kind == kDiag && (h.slot < 0 || h.gkind == kGenDiag); };
-        size_t j = i;
-        while (diag_runs && j < hops.size() && member(hops[j])) ++j;
-        int deferred = 0;
-        for (size_t t = i; t < j; ++t) {
-            const HOp &h = hops[t];
-            const bool reg = (h.cm_reg | h.pm_reg) != 0, rest = (h.f.cm_rest | h.f.pm_rest) != 0;
-            deferred += !reg || !rest || h.dneg;
-        }
-        if (deferred < 2) {
+        const size_t j = run_end(i);
+        if (j == i) {
             b.ops.push_back(pack(hops[i], b.pd.reg_off, nr));
             ++i;
             continue;
         }
         // diagonal run over hops[i, j): header, P[k] data ops, members
         if (scale_hop >= i && scale_hop < j) b.scale_members = 1;
-        std::vector<std::complex<double>> P(size_t(nr), 1.0);
-        uint32_t flags = 0;
+        std::vector<std::complex<double>> P(size_t(nr), i == fold_i ? scale : 1.0);
+        uint32_t flags = i == fold_i ? kRunHasP : 0u;
         std::vector<FOp> members;
         for (size_t t = i; t < j; ++t) {
             const HOp &h = hops[t];
