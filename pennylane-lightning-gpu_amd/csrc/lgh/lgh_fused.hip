@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void k_slot_store(const double *__restrict_
 // host form of FOp before its fields are packed into meta
 struct HOp {
     uint32_t kind = 0, gkind = 0, dneg = 0;
-    uint32_t rot = 0; // matrix [[1, -t], [t, 1]] (a rotation with its cosine factored out)
+    uint32_t rot = 0; // 1: matrix [[1, -t], [t, 1]], 2: [[t, -1], [1, t]] (a rotation with a factor taken out)
     int loc = 0, slot = -1;
     uint64_t cm_reg = 0, pm_reg = 0; // control / parity bits among the register bits
     FOp f{};
@@ -461,6 +461,7 @@ FOp pack(const HOp &h, const uint64_t (&reg_off)[kMaxNR], int nr) {
                      f.m[1].im == 0.0 && f.m[2].re == 1.0 && f.m[2].im == 0.0 && f.m[3].re == 0.0 && f.m[3].im == 0.0;
     f.meta = h.kind | uint32_t(h.loc) << 4 | uint32_t(h.slot + 1) << 8 | h.gkind << 16 | h.dneg << 20 |
              uint32_t(real) << 21 | uint32_t(dsym) << 22 | uint32_t(isx) << 23 | h.rot << 28;
+    // (bits 28-29: the rotation form)
     uint32_t rok = 0, rpar = 0;
     for (int k = 0; k < kMaxNR; ++k) {
         // registers beyond nr do not exist; their bits read as "controls hold"
@@ -707,9 +708,10 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
         hops.push_back(h);
         if (merge && single_bit >= 0) last_single[size_t(single_bit)] = int(hops.size()) - 1;
     }
-    // Rotations [[c, -s], [s, c]] with |c| >= |s| (RY, its adjoint, products
-    // of them) become c [[1, -t], [t, 1]], t = s / c: one FMA per output
-    // component instead of a multiply and an FMA.  The factors c multiply
+    // Rotations [[c, -s], [s, c]] (RY, its adjoint, products of them) become
+    // c [[1, -t], [t, 1]] with t = s / c when |c| >= |s|, else
+    // s [[t, -1], [1, t]] with t = c / s: one FMA per output component instead
+    // of a multiply and an FMA.  The factors taken out multiply
     // into one scale C for the whole pass, applied by a diagonal gate at its
     // end.  Until then the registers hold the states divided by the running
     // product, so the overlap of a gate taken there is divided by its square:
@@ -741,15 +743,25 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
             if (h.kind == kDiag || h.cm_reg != 0 || h.f.cm_rest != 0) continue;
             const cx<double> *m = h.f.m;
             if (m[0].im != 0.0 || m[1].im != 0.0 || m[2].im != 0.0 || m[3].im != 0.0) continue;
-            const double c = m[0].re;
-            if (m[3].re != c || m[2].re != -m[1].re || c == 0.0 || std::abs(c) < std::abs(m[2].re)) continue;
-            const double t = m[2].re / c;
-            h.f.m[0] = cx<double>{1.0, 0.0};
-            h.f.m[1] = cx<double>{-t, 0.0};
-            h.f.m[2] = cx<double>{t, 0.0};
-            h.f.m[3] = cx<double>{1.0, 0.0};
-            h.rot = 1;
-            scale *= c;
+            const double c = m[0].re, s = m[2].re;
+            if (m[3].re != c || m[1].re != -s || (c == 0.0 && s == 0.0)) continue;
+            if (std::abs(c) >= std::abs(s)) { // c [[1, -s/c], [s/c, 1]]
+                const double t = s / c;
+                h.f.m[0] = cx<double>{1.0, 0.0};
+                h.f.m[1] = cx<double>{-t, 0.0};
+                h.f.m[2] = cx<double>{t, 0.0};
+                h.f.m[3] = cx<double>{1.0, 0.0};
+                h.rot = 1;
+                scale *= c;
+            } else { // s [[c/s, -1], [1, c/s]]
+                const double t = c / s;
+                h.f.m[0] = cx<double>{t, 0.0};
+                h.f.m[1] = cx<double>{-1.0, 0.0};
+                h.f.m[2] = cx<double>{1.0, 0.0};
+                h.f.m[3] = cx<double>{t, 0.0};
+                h.rot = 2;
+                scale *= s;
+            }
         }
         if (scale != 1.0) {
             // The scale is a global factor, so any point of the pass may

@@ -56,9 +56,9 @@ __device__ __forceinline__ bool f_real(const FOp &o) { return (o.meta >> 21) & 1
 __device__ __forceinline__ bool f_dsym(const FOp &o) { return (o.meta >> 22) & 1u; }
 // dense 2x2 equal to [[0, 1], [1, 0]] (X, CNOT, Toffoli)
 __device__ __forceinline__ bool f_isx(const FOp &o) { return (o.meta >> 23) & 1u; }
-// dense 2x2 equal to [[1, -t], [t, 1]]: a rotation whose cosine the host
-// moved into a scale applied at the end of the pass
-__device__ __forceinline__ bool f_rot(const FOp &o) { return (o.meta >> 28) & 1u; }
+// rotation with a factor moved by the host into a scale of the whole pass:
+// 1: [[1, -t], [t, 1]] (t in m[2]), 2: [[t, -1], [1, t]] (t in m[0]), 0: neither
+__device__ __forceinline__ int f_rot(const FOp &o) { return int((o.meta >> 28) & 3u); }
 // every register satisfies the register-bit controls (the controls, if any,
 // are on lane or higher bits)
 __device__ __forceinline__ bool f_regs_ok(const FOp &o) { return (o.rbits & 0xffffu) == 0xffffu; }
@@ -177,12 +177,15 @@ template <class R> struct M2 {
 template <class R> __device__ __forceinline__ M2<R> m2(const FOp &op) {
     return M2<R>{lo<R>(op.m[0]), lo<R>(op.m[1]), lo<R>(op.m[2]), lo<R>(op.m[3])};
 }
-template <class R, bool Real, bool Rot = false>
+template <class R, bool Real, int Rot = 0>
 __device__ __forceinline__ void mat2(const M2<R> &m, cx<R> &x0, cx<R> &x1) {
     cx<R> y0, y1;
-    if constexpr (Rot) {
+    if constexpr (Rot == 1) {
         y0 = rfma(m.m01.re, x1, x0);
         y1 = rfma(m.m10.re, x0, x1);
+    } else if constexpr (Rot == 2) {
+        y0 = rfma(m.m00.re, x0, neg(x1));
+        y1 = rfma(m.m11.re, x1, x0);
     } else if constexpr (Real) {
         y0 = rfma(m.m01.re, x1, rmul(m.m00.re, x0));
         y1 = rfma(m.m11.re, x1, rmul(m.m10.re, x0));
@@ -194,9 +197,10 @@ __device__ __forceinline__ void mat2(const M2<R> &m, cx<R> &x0, cx<R> &x1) {
     x1 = y1;
 }
 // lane bit L <= 3: each lane updates its own amplitude from the partner's
-template <class R, bool Real, bool Rot = false>
+// (form 2 on lane bits takes the real path: the partner's sign would differ by lane)
+template <class R, bool Real, int Rot = 0>
 __device__ __forceinline__ cx<R> row2(cx<R> cs, cx<R> cp, cx<R> own, cx<R> partner) {
-    if constexpr (Rot)
+    if constexpr (Rot == 1)
         return rfma(cp.re, partner, own);
     else if constexpr (Real)
         return rfma(cp.re, partner, rmul(cs.re, own));
@@ -205,13 +209,13 @@ __device__ __forceinline__ cx<R> row2(cx<R> cs, cx<R> cp, cx<R> own, cx<R> partn
 }
 
 // --- uncontrolled gates -------------------------------------------------
-template <class R, int NR, int B, bool Real, bool Rot = false>
+template <class R, int NR, int B, bool Real, int Rot = 0>
 __device__ __forceinline__ void reg_plain(const M2<R> &m, cx<R> (&v)[NR]) {
 #pragma unroll
     for (int k = 0; k < NR; ++k)
         if (!(k & (1 << B))) mat2<R, Real, Rot>(m, v[k], v[k | (1 << B)]);
 }
-template <class R, int NR, int L, bool Real, bool Rot = false>
+template <class R, int NR, int L, bool Real, int Rot = 0>
 __device__ __forceinline__ void lane_plain(const M2<R> &m, int lane, cx<R> (&v)[NR]) {
     if constexpr (L >= 4) {
 #pragma unroll
@@ -224,7 +228,7 @@ __device__ __forceinline__ void lane_plain(const M2<R> &m, int lane, cx<R> (&v)[
         const bool hi = (lane >> L) & 1;
         const cx<R> cs = hi ? m.m11 : m.m00, cp = hi ? m.m10 : m.m01;
 #pragma unroll
-        for (int k = 0; k < NR; ++k) v[k] = row2<R, Real, Rot>(cs, cp, v[k], xor_lane_cx<L>(v[k], lane));
+        for (int k = 0; k < NR; ++k) v[k] = row2<R, Real, Rot == 1 ? 1 : 0>(cs, cp, v[k], xor_lane_cx<L>(v[k], lane));
     }
 }
 
@@ -357,8 +361,10 @@ template <class R, int NR, int B> __device__ __forceinline__ void reg_gate(const
                 reg_masked<R, NR, B, false, false>(op, m, lane_ok, v);
         }
     }
-    else if (f_rot(op))
-        reg_plain<R, NR, B, true, true>(m, v);
+    else if (f_rot(op) == 1)
+        reg_plain<R, NR, B, true, 1>(m, v);
+    else if (f_rot(op) == 2)
+        reg_plain<R, NR, B, true, 2>(m, v);
     else if (f_real(op))
         reg_plain<R, NR, B, true>(m, v);
     else
@@ -380,8 +386,10 @@ __device__ __forceinline__ void lane_gate(const FOp &op, int lane, bool lane_ok,
                 lane_masked<R, NR, L, false, false>(op, m, lane, lane_ok, v);
         }
     }
-    else if (f_rot(op))
-        lane_plain<R, NR, L, true, true>(m, lane, v);
+    else if (f_rot(op) == 1)
+        lane_plain<R, NR, L, true, 1>(m, lane, v);
+    else if (L >= 4 && f_rot(op) == 2)
+        lane_plain<R, NR, L, true, 2>(m, lane, v);
     else if (f_real(op))
         lane_plain<R, NR, L, true>(m, lane, v);
     else
@@ -499,7 +507,7 @@ __device__ __forceinline__ double lane_dot_masked(const FOp &op, const FGen &g, 
 // One reverse step of an uncontrolled dense gate on (lambda, bra): the
 // generator overlap (with Gen) and U^dagger on both states, sharing the lane
 // exchanges between them.
-template <class R, int NR, int B, bool Real, bool Gen, bool Rot = false>
+template <class R, int NR, int B, bool Real, bool Gen, int Rot = 0>
 __device__ __forceinline__ double adj_reg_plain(const M2<R> &m, uint32_t gk, const FGen *g, cx<R> (&a)[NR],
                                                 cx<R> (&b)[NR]) {
     double s = 0.0;
@@ -513,7 +521,7 @@ __device__ __forceinline__ double adj_reg_plain(const M2<R> &m, uint32_t gk, con
     }
     return s;
 }
-template <class R, int NR, int L, bool Real, bool Gen, bool Rot = false>
+template <class R, int NR, int L, bool Real, bool Gen, int Rot = 0>
 __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, const FGen *g, int lane,
                                                  cx<R> (&a)[NR], cx<R> (&b)[NR]) {
     double s = 0.0;
@@ -535,8 +543,8 @@ __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, co
         for (int k = 0; k < NR; ++k) {
             const cx<R> pa = xor_lane_cx<L>(a[k], lane);
             if constexpr (Gen) s += row_dot<R>(gk, *g, hi, a[k], pa, b[k]);
-            a[k] = row2<R, Real, Rot>(cs, cp, a[k], pa);
-            b[k] = row2<R, Real, Rot>(cs, cp, b[k], xor_lane_cx<L>(b[k], lane));
+            a[k] = row2<R, Real, Rot == 1 ? 1 : 0>(cs, cp, a[k], pa);
+            b[k] = row2<R, Real, Rot == 1 ? 1 : 0>(cs, cp, b[k], xor_lane_cx<L>(b[k], lane));
         }
     }
     return s;
@@ -544,16 +552,19 @@ __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, co
 
 // the three matrix forms of an uncontrolled dense step
 template <class R, int NR, int B, bool Gen>
-__device__ __forceinline__ double adj_reg_any(const M2<R> &m, bool rot, bool real, uint32_t gk, const FGen *g,
+__device__ __forceinline__ double adj_reg_any(const M2<R> &m, int rot, bool real, uint32_t gk, const FGen *g,
                                               cx<R> (&a)[NR], cx<R> (&b)[NR]) {
-    if (rot) return adj_reg_plain<R, NR, B, true, Gen, true>(m, gk, g, a, b);
+    if (rot == 1) return adj_reg_plain<R, NR, B, true, Gen, 1>(m, gk, g, a, b);
+    if (rot == 2) return adj_reg_plain<R, NR, B, true, Gen, 2>(m, gk, g, a, b);
     if (real) return adj_reg_plain<R, NR, B, true, Gen>(m, gk, g, a, b);
     return adj_reg_plain<R, NR, B, false, Gen>(m, gk, g, a, b);
 }
 template <class R, int NR, int L, bool Gen>
-__device__ __forceinline__ double adj_lane_any(const M2<R> &m, bool rot, bool real, uint32_t gk, const FGen *g,
+__device__ __forceinline__ double adj_lane_any(const M2<R> &m, int rot, bool real, uint32_t gk, const FGen *g,
                                                int lane, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
-    if (rot) return adj_lane_plain<R, NR, L, true, Gen, true>(m, gk, g, lane, a, b);
+    if (rot == 1) return adj_lane_plain<R, NR, L, true, Gen, 1>(m, gk, g, lane, a, b);
+    if constexpr (L >= 4)
+        if (rot == 2) return adj_lane_plain<R, NR, L, true, Gen, 2>(m, gk, g, lane, a, b);
     if (real) return adj_lane_plain<R, NR, L, true, Gen>(m, gk, g, lane, a, b);
     return adj_lane_plain<R, NR, L, false, Gen>(m, gk, g, lane, a, b);
 }
@@ -563,7 +574,8 @@ __device__ __forceinline__ double adj_plain_dense(const FOp &op, const FGen *g, 
                                                   cx<R> (&b)[NR]) {
     const M2<R> m = m2<R>(op);
     const uint32_t gk = f_gkind(op);
-    const bool real = f_real(op), rot = f_rot(op);
+    const bool real = f_real(op);
+    const int rot = f_rot(op);
     const int loc = f_loc(op);
     if (f_kind(op) == kRegDense) {
         switch (loc) {
