@@ -451,6 +451,24 @@ __device__ __forceinline__ double pair_dot(uint32_t gk, const FGen &g, cx<R> a0,
     const cx<double> x0 = up(a0), x1 = up(a1);
     return im_cdot(b0, cfma(g.g[1], x1, cmul(g.g[0], x0))) + im_cdot(b1, cfma(g.g[3], x1, cmul(g.g[2], x0)));
 }
+// s plus that overlap; for G = X and G = Y one chain of four FMAs
+template <class R>
+__device__ __forceinline__ double pair_dot_acc(uint32_t gk, const FGen &g, cx<R> a0, cx<R> a1, cx<R> b0, cx<R> b1,
+                                               double s) {
+    if (gk == kGenX) { // b0.re a1.im - b0.im a1.re + b1.re a0.im - b1.im a0.re
+        s = fma(double(b0.re), double(a1.im), s);
+        s = fma(-double(b0.im), double(a1.re), s);
+        s = fma(double(b1.re), double(a0.im), s);
+        return fma(-double(b1.im), double(a0.re), s);
+    }
+    if (gk == kGenY) { // b1.re a0.re + b1.im a0.im - b0.re a1.re - b0.im a1.im
+        s = fma(double(b1.re), double(a0.re), s);
+        s = fma(double(b1.im), double(a0.im), s);
+        s = fma(-double(b0.re), double(a1.re), s);
+        return fma(-double(b0.im), double(a1.im), s);
+    }
+    return s + pair_dot<R>(gk, g, a0, a1, b0, b1);
+}
 // lane bit L <= 3: one lane's row of the pair sum; own = its amplitude,
 // partner = the amplitude in lane ^ 2^L, hi = the lane's bit L
 template <class R>
@@ -515,7 +533,7 @@ __device__ __forceinline__ double adj_reg_plain(const M2<R> &m, uint32_t gk, con
     for (int k = 0; k < NR; ++k) {
         if (k & (1 << B)) continue;
         const int k1 = k | (1 << B);
-        if constexpr (Gen) s += pair_dot<R>(gk, *g, a[k], a[k1], b[k], b[k1]);
+        if constexpr (Gen) s = pair_dot_acc<R>(gk, *g, a[k], a[k1], b[k], b[k1], s);
         mat2<R, Real, Rot>(m, a[k], a[k1]);
         mat2<R, Real, Rot>(m, b[k], b[k1]);
     }
@@ -530,7 +548,7 @@ __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, co
         for (int k = 0; k < NR; k += 2) {
             row_swap_cx<L>(a[k], a[k + 1]);
             row_swap_cx<L>(b[k], b[k + 1]);
-            if constexpr (Gen) s += pair_dot<R>(gk, *g, a[k], a[k + 1], b[k], b[k + 1]);
+            if constexpr (Gen) s = pair_dot_acc<R>(gk, *g, a[k], a[k + 1], b[k], b[k + 1], s);
             mat2<R, Real, Rot>(m, a[k], a[k + 1]);
             mat2<R, Real, Rot>(m, b[k], b[k + 1]);
             row_swap_cx<L>(a[k], a[k + 1]);
