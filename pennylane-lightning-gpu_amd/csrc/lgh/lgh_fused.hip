@@ -812,6 +812,7 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
             uint32_t cls, ov = kOvF;
             if (!reg) {
                 cls = kMemQ;
+                flags |= kRunHasQ;
             } else if (!rest) {
                 cls = kMemR;
                 flags |= kRunHasP;

@@ -84,7 +84,7 @@ __device__ __forceinline__ bool f_regs_ok(const FOp &o) { return (o.rbits & 0xff
 // carries their class (bits 24-25) and overlap form (bits 26-27).
 enum : uint32_t { kMemQ = 0, kMemR = 1, kMemS = 2, kMemX = 3 };
 enum : uint32_t { kOvT = 0, kOvB = 1, kOvF = 2 };
-enum : uint32_t { kRunHasP = 1, kRunNeedsTE = 2, kRunHasS = 4 };
+enum : uint32_t { kRunHasP = 1, kRunNeedsTE = 2, kRunHasS = 4, kRunHasQ = 8 };
 __device__ __forceinline__ uint32_t f_mclass(const FOp &o) { return (o.meta >> 24) & 3u; }
 __device__ __forceinline__ uint32_t f_oclass(const FOp &o) { return (o.meta >> 26) & 3u; }
 __device__ __forceinline__ int run_nmem(const FOp &o) { return int(o.pad & 0xffffu); }
@@ -746,32 +746,55 @@ __device__ __forceinline__ uint32_t run_s(const FOp &m, uint64_t base, uint32_t 
     return sm ^ (((base & m.cm_rest) == m.cm_rest) ? (m.rbits & 0xffffu) : 0u);
 }
 // factor of register K at the end of a run
-template <class R, int K>
+// HasQ: the run has per-lane members; without them q is 1 and the factor of
+// a register is its constant P[k] (with its sign)
+template <class R, int K, bool HasQ>
 __device__ __forceinline__ cx<R> run_factor(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm) {
-    cx<R> f = q;
-    if (flags & kRunHasP) f = cmul(f, lo<R>(cx<double>{hp[1 + K / 4].m[K % 4].re, hp[1 + K / 4].m[K % 4].im}));
+    cx<R> f = HasQ ? q : cx<R>{R(1), R(0)};
+    if (flags & kRunHasP) { // the data ops exist only then
+        const cx<R> p = lo<R>(cx<double>{hp[1 + K / 4].m[K % 4].re, hp[1 + K / 4].m[K % 4].im});
+        if constexpr (HasQ)
+            f = cmul(f, p);
+        else
+            f = p;
+    }
     if (flags & kRunHasS) {
         const uint32_t fm = (sm << (31 - K)) & kSignBit;
         f = cx<R>{flip(f.re, fm), flip(f.im, fm)};
     }
     return f;
 }
-template <class R, int NR, int K = 0>
-__device__ __forceinline__ void run_finish(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR]) {
+template <class R, int NR, bool HasQ, int K = 0>
+__device__ __forceinline__ void run_finish_q(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR]) {
     if constexpr (K < NR) {
-        a[K] = cmul(a[K], run_factor<R, K>(hp, flags, q, sm));
-        run_finish<R, NR, K + 1>(hp, flags, q, sm, a);
+        a[K] = cmul(a[K], run_factor<R, K, HasQ>(hp, flags, q, sm));
+        run_finish_q<R, NR, HasQ, K + 1>(hp, flags, q, sm, a);
     }
 }
-template <class R, int NR, int K = 0>
-__device__ __forceinline__ void run_finish2(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR],
-                                            cx<R> (&b)[NR]) {
+template <class R, int NR>
+__device__ __forceinline__ void run_finish(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR]) {
+    if (flags & kRunHasQ)
+        run_finish_q<R, NR, true>(hp, flags, q, sm, a);
+    else
+        run_finish_q<R, NR, false>(hp, flags, q, sm, a);
+}
+template <class R, int NR, bool HasQ, int K = 0>
+__device__ __forceinline__ void run_finish2_q(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR],
+                                              cx<R> (&b)[NR]) {
     if constexpr (K < NR) {
-        const cx<R> f = run_factor<R, K>(hp, flags, q, sm);
+        const cx<R> f = run_factor<R, K, HasQ>(hp, flags, q, sm);
         a[K] = cmul(a[K], f);
         b[K] = cmul(b[K], f);
-        run_finish2<R, NR, K + 1>(hp, flags, q, sm, a, b);
+        run_finish2_q<R, NR, HasQ, K + 1>(hp, flags, q, sm, a, b);
     }
+}
+template <class R, int NR>
+__device__ __forceinline__ void run_finish2(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR],
+                                            cx<R> (&b)[NR]) {
+    if (flags & kRunHasQ)
+        run_finish2_q<R, NR, true>(hp, flags, q, sm, a, b);
+    else
+        run_finish2_q<R, NR, false>(hp, flags, q, sm, a, b);
 }
 
 // Forward: the run whose header hdr is at hp.
