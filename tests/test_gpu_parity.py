@@ -540,6 +540,39 @@ def test_fused_random_sweep_vs_oracle(seed):
     close64(got64, ref)
 
 
+def test_fused_rotation_forms_special_angles():
+    """Real rotations are stored as c [[1, -t], [t, 1]] or s [[t, -1], [1, t]]
+    (whichever of cos, sin is larger) with the factors applied once per pass;
+    the Jacobian factors are rescaled by the running product.  Angles at the
+    boundaries of that choice: |cos| = |sin| (theta = pi/2), cos ~ 6e-17
+    (theta = pi), the identity (theta = 0), negative cosines, and chains of
+    rotations on one wire that the forward passes merge."""
+    rng = np.random.default_rng(77)
+    n = 12
+    angles = [0.0, np.pi / 2, -np.pi / 2, np.pi, -np.pi, 3 * np.pi / 2, 2.5, -2.9, 4.0, 1e-9]
+    ops = []
+    for layer in range(3):
+        for w in range(n):
+            th = float(angles[(w + 3 * layer) % len(angles)])
+            ops.append(("RY", [w], [th], bool((w + layer) % 3 == 0)))
+            ops.append(("RZ", [w], [float(rng.uniform(-3, 3))], False))
+            if w % 4 == 1:
+                ops.append(("RY", [w], [float(angles[(w + layer + 5) % len(angles)])], False))
+        for w in range(layer % 2, n - 1, 2):
+            ops.append(("CZ", [w, w + 1], [], False))
+    init = rand_state(n, rng)
+    psi = po.apply_ops(n, init, ops)
+    close128(N.State(n).upload(init).apply(ops).download(), psi)
+    close64(N.State(n, np.complex64).upload(init.astype(np.complex64)).apply(ops).download(), psi)
+    bras = obs_states(n, psi, [[("PauliZ", [0]), ("PauliZ", [n - 1])], [("PauliX", [5])]])
+    tp = list(range(sum(1 for o in ops if o[2])))
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    close128(N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp), ref)
+    got64 = N.adjoint_jacobian(N.State(n, np.complex64).upload(psi.astype(np.complex64)),
+                               [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
+    close64(got64, ref)
+
+
 def test_fused_adjoint_config2_shape():
     # RY/RZ + CZ brick (the 28-qubit benchmark circuit family) at 14 qubits
     from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
