@@ -131,6 +131,12 @@ lgh_status lgh_set_fusion(int enabled);
  * of the adjoint passes; a wavefront holds 2^(6 + bits) amplitudes per state.
  * Defaults 4 and 3.  No counterpart in the reference; a tuning knob. */
 lgh_status lgh_set_tile_bits(int forward, int adjoint);
+/* Caps on the wavefronts of the fused passes' grids, forward and adjoint
+ * (0: the defaults, up to 4096 and 256 wavefronts per CU).  Each wavefront
+ * steps through ceil(tiles / wavefronts) consecutive tiles of 2^(6 + bits)
+ * amplitudes, so a small cap exercises the multi-tile path at any size.  No
+ * counterpart in the reference; a tuning knob used by tests. */
+lgh_status lgh_set_pass_waves(int64_t forward, int64_t adjoint);
 /* Plan of the fused passes that `ops` get on `num_qubits` qubits, computed on
  * the host without touching a device: passes in total, of those the
  * standalone ones (gates that fit no fused pass), and over the fused passes

@@ -91,6 +91,25 @@ template <class T> std::vector<std::complex<T>> to_vec(const py::array_t<std::co
     return std::vector<std::complex<T>>(p, p + b.size);
 }
 
+// GenerateSamples returns a (shots, num_wires) array of the state's sample
+// bits (Bindings.cpp:536-553), so num_wires must be the state's qubit count:
+// any other value would describe a buffer of a different size.
+template <class SV> void check_sample_wires(const SV &sv, std::size_t num_wires) {
+    if (num_wires != sv.getNumQubits())
+        throw LightningException("GenerateSamples: num_wires (" + std::to_string(num_wires) +
+                                 ") must equal the number of qubits of the state (" +
+                                 std::to_string(sv.getNumQubits()) + ")");
+}
+
+py::array_t<std::size_t> samples_array(const std::vector<std::size_t> &bits, std::size_t shots,
+                                       std::size_t num_wires) {
+    if (bits.size() != shots * num_wires)
+        throw LightningException("GenerateSamples: sample buffer does not match (shots, num_wires)");
+    py::array_t<std::size_t> out({shots, num_wires});
+    if (!bits.empty()) std::memcpy(out.mutable_data(), bits.data(), bits.size() * sizeof(std::size_t));
+    return out;
+}
+
 template <class PrecisionT, class ParamT> void bind_precision(py::module_ &m) {
     using SV = StateVectorHIPManaged<PrecisionT>;
     using CT = std::complex<PrecisionT>;
@@ -207,17 +226,13 @@ template <class PrecisionT, class ParamT> void bind_precision(py::module_ &m) {
             "Calculate the probabilities for given wires. Results returned in Col-major order.")
         .def("GenerateSamples",
              [](SV &sv, std::size_t num_wires, std::size_t num_shots) {
-                 auto r = sv.generate_samples(num_shots);
-                 py::array_t<std::size_t> out({num_shots, num_wires});
-                 std::memcpy(out.mutable_data(), r.data(), r.size() * sizeof(std::size_t));
-                 return out;
+                 check_sample_wires(sv, num_wires);
+                 return samples_array(sv.generate_samples(num_shots), num_shots, num_wires);
              })
         .def("GenerateSamples",
              [](SV &sv, std::size_t num_wires, std::size_t num_shots, uint64_t seed) {
-                 auto r = sv.generate_samples(num_shots, seed);
-                 py::array_t<std::size_t> out({num_shots, num_wires});
-                 std::memcpy(out.mutable_data(), r.data(), r.size() * sizeof(std::size_t));
-                 return out;
+                 check_sample_wires(sv, num_wires);
+                 return samples_array(sv.generate_samples(num_shots, seed), num_shots, num_wires);
              })
         .def("GenerateSampleIndices",
              [](SV &sv, std::size_t num_shots, uint64_t seed) {

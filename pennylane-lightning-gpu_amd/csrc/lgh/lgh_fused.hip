@@ -45,6 +45,8 @@ std::atomic<int> g_fusion{1};
 // floats), whole cache lines; the other lane positions are chosen per pass.
 int fixed_lanes(lgh_dtype dt) { return dt == LGH_C128 ? 3 : 4; }
 std::atomic<int> g_rb_fwd{kRBForward}, g_rb_adj{kRBAdjoint};
+// caps on the wavefronts of a pass's grid (0: the per-CU defaults below)
+std::atomic<int64_t> g_waves_fwd{0}, g_waves_adj{0};
 
 struct Use {
     uint64_t diag = 0;    // bits used diagonally (controls, phases)
@@ -71,6 +73,11 @@ void set_tile_bits(int forward, int adjoint) {
     g_rb_adj.store(adjoint);
 }
 int fusion_enabled() { return g_fusion.load(); }
+void set_pass_waves(int64_t forward, int64_t adjoint) {
+    LGH_REQUIRE(forward >= 0 && adjoint >= 0, "wavefront caps are >= 0 (0: default)");
+    g_waves_fwd.store(forward);
+    g_waves_adj.store(adjoint);
+}
 
 bool fusable(const KOp &k, int n, int rb) {
     if (n < kLaneBits + rb) return false;
@@ -543,7 +550,7 @@ void cluster_diagonals(std::vector<int> &order, const std::vector<KOp> &kops, in
 
 Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse,
                           const std::vector<KGen> *gens, const std::vector<int> *slot_of, int rb, bool merge_forward,
-                          bool diag_runs) {
+                          bool diag_runs, bool single) {
     LGH_REQUIRE(rb >= 1 && rb <= kMaxRB, "fused pass: bad register-bit count");
     const int nr = 1 << rb;
     Built b{};
@@ -735,6 +742,10 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
     size_t scale_hop = hops.size() + 1;           // index of a closing scale gate, if any
     size_t fold_i = hops.size() + 1, fold_j = 0;  // the run that applies the scale, if any
     double scale = 1.0;
+    // smallest running product: 2^-20 for complex64 registers (values up to
+    // 2^20 times the amplitudes, far inside float range), 2^-100 for
+    // complex128 (the squared ratios of the Jacobian factors stay normal)
+    const double scale_floor = single ? 0x1p-20 : 0x1p-100;
     if (diag_runs) {
         std::vector<double> before(hops.size(), 1.0); // running scale before each gate
         for (size_t x = 0; x < hops.size(); ++x) {
@@ -745,6 +756,12 @@ Built lgh::fz::build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bo
             if (m[0].im != 0.0 || m[1].im != 0.0 || m[2].im != 0.0 || m[3].im != 0.0) continue;
             const double c = m[0].re, s = m[2].re;
             if (m[3].re != c || m[1].re != -s || (c == 0.0 && s == 0.0)) continue;
+            // The factor taken out is the larger of |c|, |s| (at least 2^-1/2),
+            // so the running product only shrinks.  The registers hold the
+            // states divided by it: past the floor they could overflow (and
+            // the closing factor underflow) in the working precision, so
+            // the rest of the pass keeps its rotations whole.
+            if (std::abs(scale * (std::abs(c) >= std::abs(s) ? c : s)) < scale_floor) continue;
             if (std::abs(c) >= std::abs(s)) { // c [[1, -s/c], [s/c, 1]]
                 const double t = s / c;
                 h.f.m[0] = cx<double>{1.0, 0.0};
@@ -871,10 +888,12 @@ struct Geometry {
 constexpr int kWavesPerCUForward = 4096; // one tile per wavefront up to 30 qubits
 constexpr int kWavesPerCUAdjoint = 256;
 
-Geometry geometry(int n, int device, int rb, int waves_per_cu) {
+// cap: at most that many wavefronts (0: no cap beyond waves_per_cu)
+Geometry geometry(int n, int device, int rb, int waves_per_cu, int64_t cap = 0) {
     Geometry g{};
     g.ntiles = 1ull << (n - kLaneBits - rb);
-    const uint64_t max_waves = uint64_t(std::max(1, sm_count(device))) * uint64_t(waves_per_cu);
+    uint64_t max_waves = uint64_t(std::max(1, sm_count(device))) * uint64_t(waves_per_cu);
+    if (cap > 0) max_waves = std::min<uint64_t>(max_waves, uint64_t(cap));
     uint64_t waves = std::min<uint64_t>(g.ntiles, max_waves);
     g.chunk = (g.ntiles + waves - 1) / waves;
     waves = (g.ntiles + g.chunk - 1) / g.chunk;
@@ -886,13 +905,13 @@ Geometry geometry(int n, int device, int rb, int waves_per_cu) {
 template <class R>
 void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pass> &passes, int rb) {
     StreamCtx *ctx = sv->ctx;
-    const Geometry geo = geometry(sv->n, sv->device, rb, kWavesPerCUForward);
+    const Geometry geo = geometry(sv->n, sv->device, rb, kWavesPerCUForward, g_waves_fwd.load());
     for (const Pass &p : passes) {
         if (p.standalone) {
             launch_kop(sv, kops[size_t(p.ops[0])]);
             continue;
         }
-        Built b = build_pass(p, kops, sv->n, false, nullptr, nullptr, rb, true, true);
+        Built b = build_pass(p, kops, sv->n, false, nullptr, nullptr, rb, true, true, sv->dtype == LGH_C64);
         const size_t pb = (sizeof(PassDesc) + 255) & ~size_t(255);
         std::vector<unsigned char> blob(pb + b.ops.size() * sizeof(FOp));
         std::memcpy(blob.data(), &b.pd, sizeof(PassDesc));
@@ -946,8 +965,9 @@ FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, 
             ++plan.standalone;
             continue;
         }
-        const Built b = adjoint ? build_pass(p, kops, n, true, &gens, &slot, rb, true, true)
-                                : build_pass(p, kops, n, false, nullptr, nullptr, rb, true, true);
+        const bool single = dtype == LGH_C64;
+        const Built b = adjoint ? build_pass(p, kops, n, true, &gens, &slot, rb, true, true, single)
+                                : build_pass(p, kops, n, false, nullptr, nullptr, rb, true, true, single);
         for (size_t j = 0; j < b.ops.size();) {
             const FOp &f = b.ops[j];
             if ((f.meta & 15u) != kDiagRun) {
@@ -975,7 +995,7 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
     const int n = lam->n;
     const int rb = g_rb_adj.load();
     const std::vector<Pass> passes = schedule_passes(udag, n, slot_col, rb, fixed_lanes(lam->dtype));
-    const Geometry geo = geometry(n, lam->device, rb, kWavesPerCUAdjoint);
+    const Geometry geo = geometry(n, lam->device, rb, kWavesPerCUAdjoint, g_waves_adj.load());
     // lambda still to be read from lam_src (until the first pass has run)
     const void *src = lam_src;
     for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
@@ -990,7 +1010,7 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
                        factor[size_t(i)]);
             continue;
         }
-        Built b = build_pass(p, udag, n, true, &gens, &slot_col, rb, true, true);
+        Built b = build_pass(p, udag, n, true, &gens, &slot_col, rb, true, true, lam->dtype == LGH_C64);
         std::vector<int64_t> cols;
         std::vector<double> facs;
         for (size_t s = 0; s < b.slot_ops.size(); ++s) {
@@ -1053,6 +1073,10 @@ extern "C" lgh_status lgh_set_fusion(int enabled) {
 
 extern "C" lgh_status lgh_set_tile_bits(int forward, int adjoint) {
     return guard([&] { set_tile_bits(forward, adjoint); });
+}
+
+extern "C" lgh_status lgh_set_pass_waves(int64_t forward, int64_t adjoint) {
+    return guard([&] { set_pass_waves(forward, adjoint); });
 }
 
 extern "C" lgh_status lgh_fusion_plan(int32_t num_qubits, const lgh_op *ops, int64_t num_ops, int32_t adjoint,

@@ -26,6 +26,8 @@ void set_fusion(int enabled);
 // register bits of forward and adjoint passes (3 or 4 each)
 void set_tile_bits(int forward, int adjoint);
 int fusion_enabled();
+// caps on the wavefronts of the forward and adjoint passes' grids (0: default)
+void set_pass_waves(int64_t forward, int64_t adjoint);
 // rb: register bits of the passes (tile = 2^(6 + rb) amplitudes per wavefront)
 bool fusable(const KOp &k, int n, int rb);
 // trainable_slot[i] >= 0 marks gates whose generator overlap is accumulated

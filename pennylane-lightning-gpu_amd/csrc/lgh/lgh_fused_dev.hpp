@@ -902,10 +902,13 @@ struct Built {
 // trainable gates (nullptr in the forward direction, which instead merges
 // consecutive single-qubit gates on one bit unless merge is false; without
 // merging, ops[j] is the j-th gate of the pass in execution order).
-// diag_runs: group consecutive diagonal gates into diagonal runs (the
-// single-state kernels; the batched kernels take plain gates).
+// diag_runs: group consecutive diagonal gates into diagonal runs and take
+// the factors out of real rotations (the single-state kernels; the batched
+// kernels take plain gates).  single: the pass runs on complex64 states,
+// which bounds how far the rotation factors may shrink the registers' scale.
 Built build_pass(const Pass &p, const std::vector<KOp> &kops, int n, bool reverse, const std::vector<KGen> *gens,
-                 const std::vector<int> *slot_of, int rb, bool merge = true, bool diag_runs = false);
+                 const std::vector<int> *slot_of, int rb, bool merge = true, bool diag_runs = false,
+                 bool single = false);
 
 } // namespace fz
 } // namespace lgh

@@ -111,6 +111,7 @@ def lib():
         L.lgh_stream_destroy.argtypes = [vp]
         L.lgh_stream_synchronize.argtypes = [vp]
         L.lgh_set_tile_bits.argtypes = [C.c_int, C.c_int]
+        L.lgh_set_pass_waves.argtypes = [C.c_int64, C.c_int64]
         L.lgh_fusion_plan.argtypes = [C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int] + [C.POINTER(C.c_int32)] * 4
         L.lgh_profile_enable.argtypes = [C.c_int]
         L.lgh_profile_read.argtypes = [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
@@ -143,6 +144,12 @@ def set_fusion(enabled: bool):
 def set_tile_bits(forward: int, adjoint: int):
     """Register bits (3 or 4) of the fused forward and adjoint passes."""
     check(lib().lgh_set_tile_bits(int(forward), int(adjoint)))
+
+
+def set_pass_waves(forward: int, adjoint: int):
+    """Caps on the wavefronts of the fused passes' grids (0: the defaults);
+    each wavefront then steps through a run of consecutive tiles."""
+    check(lib().lgh_set_pass_waves(int(forward), int(adjoint)))
 
 
 def fusion_plan(n, ops, adjoint=False, dtype=np.complex128):

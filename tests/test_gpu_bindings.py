@@ -131,6 +131,22 @@ def test_probability_and_samples():
     np.testing.assert_array_equal(s1, s2)
 
 
+@pytest.mark.parametrize("cls", [C128, C64])
+def test_generate_samples_rejects_other_widths(cls):
+    """The (shots, num_wires) result only describes the state's samples when
+    num_wires is its qubit count (Bindings.cpp:536-553); other widths raise
+    instead of writing past the array."""
+    sv = cls(4)
+    sv.Hadamard([1], False, [])
+    for bad in (0, 1, 3, 5, 64):
+        with pytest.raises(ops_mod.PLException):
+            sv.GenerateSamples(bad, 100)
+        with pytest.raises(ops_mod.PLException):
+            sv.GenerateSamples(bad, 100, 7)
+    s = sv.GenerateSamples(4, 100, 7)
+    assert s.shape == (100, 4) and np.all(s[:, [0, 2, 3]] == 0)
+
+
 def obs(cls_prefix, kind, *args):
     return getattr(ops_mod, f"{kind}GPU_{cls_prefix}")(*args)
 
@@ -206,3 +222,55 @@ def test_observable_repr_and_eq():
     assert repr(T) == "PauliX[3] @ RX[1]"
     with pytest.raises(ops_mod.PLException):
         obs("C128", "TensorProdObs", [a, b])
+
+
+def random_hermitian(dim, rng):
+    A = rng.normal(size=(dim, dim)) + 1j * rng.normal(size=(dim, dim))
+    return A + A.conj().T
+
+
+@pytest.mark.parametrize("prefix", ["C128", "C64"])
+def test_adjoint_hermitian_and_sparse_vs_oracle(prefix):
+    """Adjoint Jacobians of non-diagonal HermitianObs on 2 and 3 wires (given
+    in an order other than ascending) and of a SparseHamiltonian over all
+    wires against the oracle's adjoint with bra = O psi
+    (ObservablesGPU.hpp:134-180, :469-587; AdjointDiffGPU.hpp:499-596)."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(3030)
+    n = 6
+    ops = []
+    for layer in range(2):
+        for w in range(n):
+            ops.append(("RX", [w], [float(rng.uniform(-3, 3))], False))
+            ops.append(("RY", [w], [float(rng.uniform(-3, 3))], bool(w % 2)))
+        for w in range(n - 1):
+            ops.append(("CNOT", [w, w + 1], [], False))
+        ops.append(("CRZ", [layer, n - 1 - layer], [float(rng.uniform(-3, 3))], False))
+        ops.append(("IsingXX", [1, 4], [float(rng.uniform(-3, 3))], False))
+    psi = po.apply_ops(n, po.zero_state(n), ops)
+    npar = sum(1 for o in ops if o[2])
+    tp = list(range(0, npar, 2)) + [npar - 1]
+    h2 = random_hermitian(4, rng)
+    h3 = random_hermitian(8, rng)
+    Hs = sp.random(1 << n, 1 << n, density=0.08, random_state=5, format="csr", dtype=np.float64)
+    Hs = (Hs + 1j * sp.random(1 << n, 1 << n, density=0.08, random_state=6, format="csr")).tocsr()
+    Hs = (Hs + Hs.conj().T).tocsr()
+    Hs.sort_indices()
+    cdt = np.complex128 if prefix == "C128" else np.complex64
+    it = np.int64 if prefix == "C128" else np.int32
+    observables = [
+        obs(prefix, "HermitianObs", h2.astype(cdt).ravel(), [4, 1]),
+        obs(prefix, "HermitianObs", h3.astype(cdt).ravel(), [2, 5, 0]),
+        obs(prefix, "SparseHamiltonian", Hs.data.astype(cdt), Hs.indices.astype(it), Hs.indptr.astype(it),
+            list(range(n))),
+    ]
+    bras = [po.apply_ops(n, psi, [("QubitUnitary", [4, 1], [], False, h2)]),
+            po.apply_ops(n, psi, [("QubitUnitary", [2, 5, 0], [], False, h3)]),
+            Hs @ psi]
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    got = run_adjoint(n, ops, observables, tp, prefix=prefix)
+    if prefix == "C128":
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * max(1.0, np.max(np.abs(ref))))
+    else:
+        np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * max(1.0, np.max(np.abs(ref))))

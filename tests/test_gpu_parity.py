@@ -610,3 +610,74 @@ def test_fused_tile_bits_variants(bits):
         close64(got64, ref)
     finally:
         N.set_tile_bits(4, 3)
+
+
+@pytest.mark.parametrize("chunk", [2, 3, 8])
+@pytest.mark.parametrize("n", [14, 17])
+def test_fused_multitile_wavefronts(n, chunk):
+    """Fused passes whose wavefronts step through `chunk` consecutive tiles
+    (the launch shape of 26-30-qubit states), forced at 14 and 17 qubits by
+    capping the grids' wavefronts; with chunk 3 the tile count is no multiple
+    of the chunk, so the last wavefront takes a shorter run.  Forward state
+    and adjoint Jacobian against the oracle, complex128 and complex64."""
+    from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
+
+    rng = np.random.default_rng(5100 + 10 * n + chunk)
+    ops = ry_rz_cz_brick(n, 3, seed=int(rng.integers(1 << 30))) + controlled_fusable_circuit(n, rng, 60)
+    init = rand_state(n, rng)
+    psi = po.apply_ops(n, init, ops)
+    bras = obs_states(n, psi, [[("PauliZ", [0]), ("PauliZ", [n - 1])], [("PauliX", [n // 2])]])
+    npar = sum(1 for o in ops if o[2])
+    tp = list(range(npar))
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    # tiles per pass: 2^(n - 6 - register bits); register bits 4 forward, 3 adjoint
+    fwd_tiles, adj_tiles = 1 << (n - 10), 1 << (n - 9)
+    cap_f, cap_a = -(-fwd_tiles // chunk), -(-adj_tiles // chunk)
+    assert -(-fwd_tiles // cap_f) == chunk and -(-adj_tiles // cap_a) == chunk
+    if chunk == 3:
+        assert adj_tiles % chunk != 0
+    try:
+        N.set_pass_waves(cap_f, cap_a)
+        close128(N.State(n).upload(init).apply(ops).download(), psi)
+        close64(N.State(n, np.complex64).upload(init.astype(np.complex64)).apply(ops).download(), psi)
+        got = N.adjoint_jacobian(N.State(n).upload(psi), [N.State(n).upload(b) for b in bras], ops, tp)
+        close128(got, ref)
+        got64 = N.adjoint_jacobian(N.State(n, np.complex64).upload(psi.astype(np.complex64)),
+                                   [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
+        close64(got64, ref)
+    finally:
+        N.set_pass_waves(0, 0)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_fused_deep_single_pass_rotation_scale(dtype):
+    """A 10-qubit circuit of 40 layers of RY(~pi/2) + CZ brick is one forward
+    pass holding 400 real rotations, each of which takes a factor of at most
+    2^-1/2 out of the registers.  The running product is held above a floor
+    (2^-20 for complex64, 2^-100 for complex128) by keeping later rotations
+    whole; without it the registers overflow in complex64 and the closing
+    factor underflows in complex128.  Forward state and a Jacobian over few
+    trainable gates (so that the reverse passes are long as well) against
+    the oracle."""
+    n, layers = 10, 40
+    rng = np.random.default_rng(4242)
+    ops = []
+    for _ in range(layers):
+        for w in range(n):
+            ops.append(("RY", [w], [float(np.pi / 2 + rng.uniform(-0.05, 0.05))], False))
+        for w in range(0, n - 1, 2):
+            ops.append(("CZ", [w, w + 1], [], False))
+        for w in range(1, n - 1, 2):
+            ops.append(("CZ", [w, w + 1], [], False))
+    assert N.fusion_plan(n, ops, dtype=dtype)["passes"] == 1
+    init = rand_state(n, rng)
+    psi = po.apply_ops(n, init, ops)
+    close = close128 if dtype == np.complex128 else close64
+    close(N.State(n, dtype).upload(init.astype(dtype)).apply(ops).download(), psi)
+    bras = obs_states(n, psi, [[("PauliZ", [0]), ("PauliZ", [n - 1])], [("PauliX", [3])]])
+    tp = [0, 7, 150, 333, 399]
+    ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+    got = N.adjoint_jacobian(N.State(n, dtype).upload(psi.astype(dtype)),
+                             [N.State(n, dtype).upload(b.astype(dtype)) for b in bras], ops, tp)
+    assert np.all(np.isfinite(got))
+    close(got, ref)
