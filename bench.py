@@ -48,7 +48,6 @@ def parse():
     ap.add_argument("--sweep-reps", type=int, default=10)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-ops", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1337)
     ap.add_argument("--tile-bits", type=int, nargs=2, default=None, metavar=("FWD", "ADJ"),
                     help="register bits of the fused forward / adjoint passes (library default 4 3)")
@@ -336,6 +335,127 @@ def bench_serving(N, args):
     }
 
 
+def cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
+def cpu_baseline(args, ops, n):
+    """The CPU oracle (the checker: a C/OpenMP restatement of the reference's
+    CPU path) timed on this host, on rank 0 only.
+
+    - The plain 12-qubit configuration, run whole: |0..0>, 192 ops, the 12
+      expectation values <Z_w>, the 12 x 144 adjoint Jacobian.
+    - The headline workload, summed from measured components: allocating and
+      zeroing the state, one forward pass per op class (RY, RZ, CZ), and the
+      adjoint sweep's setup and reverse step per op class, each timed at the
+      workload's own size and multiplied by that class's op count.  A full
+      28-qubit run takes minutes; the component sum takes about 20-40 s.
+    """
+    import numpy as np
+
+    from oracle import pyoracle as po
+    from pennylane_lightning_gpu.workloads import layered_ring
+
+    threads = po.num_threads()
+    out = {"unit": "s", "cores": threads, "kind": "port", "cpu_model": cpu_info(),
+           "binding": os.environ.get("OMP_PROC_BIND", "unset") + "/" + os.environ.get("OMP_PLACES", "unset")}
+    # plain configuration, whole
+    c1 = layered_ring(12, 4, seed=args.seed)
+    t0 = time.perf_counter()
+    psi = po.apply_ops(12, po.zero_state(12), c1)
+    zs = [po.expval_matrix(12, psi, [w], np.diag([1.0, -1.0])).real for w in range(12)]
+    bras = [po.apply_ops(12, psi, [("PauliZ", [w], [], False)]) for w in range(12)]
+    po.adjoint_jacobian(12, psi, bras, c1, list(range(144)))
+    out["config1"] = {"workload": "12 qubits c128, 4 x (RX, RY, RZ per wire; CNOT ring), 12 <Z_w>, 12 x 144 adjoint",
+                      "cpu_s": round(time.perf_counter() - t0, 4), "checksum": float(np.sum(zs))}
+    # headline workload by components
+    log(f"[bench] CPU oracle: {n}q components")
+    counts = {}
+    for o in ops:
+        counts[o[0]] = counts.get(o[0], 0) + 1
+    th, k = 0.731, 4
+
+    def timed(fn):
+        t = time.perf_counter()
+        r = fn()
+        return time.perf_counter() - t, r
+
+    small = po.zero_state(16)  # loads the library and its thread pool
+    po.adjoint_jacobian(16, small, [small], [("RY", [0], [th], False)], [0])
+    t_zero, psi = timed(lambda: po.zero_state(n))
+    # k gates of each class on wires spread over the register (a gate's cost
+    # depends on its target's stride)
+    spread = [round(i * (n - 2) / (k - 1)) for i in range(k)]
+    gate = {"RY": [("RY", [w], [th], False) for w in spread], "RZ": [("RZ", [w], [th], False) for w in spread],
+            "CZ": [("CZ", [w, w + 1], [], False) for w in spread]}
+    # forward: apply_ops copies the state, then applies the ops; k ops
+    # against none isolates the per-op cost
+    t_copy = sorted(timed(lambda: po.apply_ops(n, psi, []))[0] for _ in range(3))[1]
+    fwd = {g: max(0.0, (timed(lambda: po.apply_ops(n, psi, seq))[0] - t_copy) / k) for g, seq in gate.items()}
+    # adjoint: one trainable RY alone, then followed by k ops of a class (all
+    # trainable): the difference is k reverse steps of that class
+    bra = po.apply_ops(n, psi, [("PauliZ", [0], [], False), ("PauliZ", [n - 1], [], False)])
+    lead = ("RY", [0], [th], False)
+    t1 = sorted(timed(lambda: po.adjoint_jacobian(n, psi, [bra], [lead], [0]))[0] for _ in range(2))[0]
+    rev = {}
+    for g, seq0 in gate.items():
+        seq = [lead] + seq0
+        tp = list(range(sum(1 for o in seq if o[2])))
+        rev[g] = max(0.0, (timed(lambda: po.adjoint_jacobian(n, psi, [bra], seq, tp))[0] - t1) / k)
+    setup = max(0.0, t1 - rev["RY"])
+    del psi, bra
+    total = t_zero + t_copy + setup + sum(counts[g] * (fwd[g] + rev[g]) for g in counts)
+    out["value"] = round(total, 2)
+    out["components"] = {"zero_state_s": round(t_zero, 3), "forward_copy_s": round(t_copy, 3),
+                         "adjoint_setup_s": round(setup, 3),
+                         "forward_s_per_op": {g: round(v, 4) for g, v in fwd.items()},
+                         "reverse_s_per_op": {g: round(v, 4) for g, v in rev.items()}, "op_counts": counts}
+    out["sample"] = (f"{n}q c128 headline summed from components timed at {n} qubits: state zeroing, forward pass "
+                     f"and adjoint reverse step per op class (RY, RZ, CZ; four ops each on spread wires), adjoint setup; "
+                     f"{len(ops)} ops. Plain 12q configuration timed whole.")
+    return out
+
+
+def gpu_config1(ops_mod, args):
+    """The plain 12-qubit configuration on the GPU through the pybind11 API,
+    as the device runs it: forward, the 12 <Z_w>, the 12 x 144 adjoint
+    Jacobian; median of 5 after a warm-up."""
+    import numpy as np
+
+    from pennylane_lightning_gpu.workloads import layered_ring
+
+    c1 = layered_ring(12, 4, seed=args.seed)
+    sv = ops_mod.LightningGPU_C128(12)
+    adj = ops_mod.AdjointJacobianGPU_C128()
+    obs = [ops_mod.NamedObsGPU_C128("PauliZ", [w]) for w in range(12)]
+    opsl = adj.create_ops_list([o[0] for o in c1], [np.array(o[2], dtype=np.float64) for o in c1],
+                               [o[1] for o in c1], [o[3] for o in c1], [np.array([], complex) for _ in c1])
+    z = np.diag([1.0, -1.0]).astype(np.complex128).ravel()
+
+    def run():
+        sv.resetGPU()
+        sv.apply([o[0] for o in c1], [o[1] for o in c1], [o[3] for o in c1], [o[2] for o in c1])
+        [sv.ExpectationValue("PauliZ", [w], [], z) for w in range(12)]
+        return adj.adjoint_jacobian(sv, obs, opsl, list(range(144)))
+
+    run()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    return round(sorted(ts)[2], 5)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -455,23 +575,11 @@ def main():
                  "min_frac_of_peak": round(min(gbs) / HBM_PEAK_GBS, 3), "per_target": rows}
         del s
 
-    # ---------------------------------------------------------------- CPU oracle baseline (bounded sample)
+    # ---------------------------------------------------------------- CPU oracle baseline (measured components)
     cpu = None
     if not args.no_cpu:
-        from oracle import pyoracle as po
-
-        k = args.cpu_sample_ops
-        sample_ops = ops[:k]
-        log(f"[bench] CPU oracle sample: {n}q, first {k} ops, forward + adjoint")
-        t0 = time.perf_counter()
-        cpsi = po.apply_ops(n, po.zero_state(n), sample_ops)
-        cbra = po.apply_ops(n, cpsi, [("PauliZ", [0], [], False), ("PauliZ", [n - 1], [], False)])
-        cjac = po.adjoint_jacobian(n, cpsi, [cbra], sample_ops, list(range(sum(1 for o in sample_ops if o[2]))))
-        tcpu = time.perf_counter() - t0
-        del cpsi, cbra
-        cpu = {"value": round(tcpu * len(ops) / k, 2), "unit": "s", "cores": po.num_threads(), "kind": "port",
-               "sample": f"{n}q complex128, first {k} of {len(ops)} ops (forward + adjoint) took {tcpu:.2f} s; "
-                         f"scaled by {len(ops)}/{k} ops"}
+        cpu = cpu_baseline(args, ops, n)
+        cpu["config1"]["gpu_s"] = gpu_config1(ops_mod, args)
 
     rec = {
         "metric": "28q adjoint-Jacobian s",

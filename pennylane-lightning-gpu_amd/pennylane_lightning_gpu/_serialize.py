@@ -1,172 +1,188 @@
-"""Serialization of tapes into the objects of ``lightning_gpu_qubit_ops``.
+"""Tape serialization for the adjoint Jacobian of ``LightningGPU``.
 
-Same functions, argument meaning and results as the reference's
-pennylane_lightning_gpu/_serialize.py:142-336 (observables and operations for
-the adjoint Jacobian), with the backend module passed in instead of imported:
-``backend`` is the pybind11 module (or a stand-in with the same class names).
+``TapeSerializer`` holds the backend classes of one precision and the
+device's wire map, and turns a tape into the arguments of
+``AdjointJacobianGPU_*.create_ops_list`` and ``adjoint_jacobian``:
 
-- Observables: named, tensor, Hamiltonian (split into one single-term
-  Hamiltonian per term unless ``use_splitting`` is False; the device sums the
-  rows back), SparseHamiltonian (CSR), Pauli sentences.  Hermitian
-  observables are rejected for the adjoint method, as in the reference.
-- Operations: StatePrep / BasisState are skipped (and reported); Rot expands
-  into RZ, RY, RZ; an adjoint becomes its base with the inverse flag; a gate
-  without a named method on the state-vector class goes by its matrix (which
-  already includes the adjoint, so its inverse flag is cleared).
+- ``observables(tape)``: backend observable objects for all tape
+  observables, flattened, with the offset of each tape observable's first
+  row.  A sum (Hamiltonian, or a Pauli sentence) gives one single-term
+  Hamiltonian per term when sums are split, so the adjoint sweep computes
+  one Jacobian row per term and the device adds the rows back.
+- ``operations(tape)``: the five OpsData lists -- names, parameter arrays,
+  wires, inverse flags, matrices -- and whether the tape holds a state
+  preparation.
+
+The objects built and their arguments are those of the reference's
+serializer (pennylane_lightning_gpu/_serialize.py:142-336), which is the
+wire format of the pybind11 boundary:
+
+- named observables take mapped wires, Identity only its first wire;
+- a tensor product is built from its factors, a Pauli word {wire: letter}
+  from one named observable per letter (the named observable itself for a
+  single letter);
+- a sparse Hamiltonian is passed as CSR (data, indices, indptr) in the
+  precision's complex and index types, with its wires mapped;
+- Hermitian observables are refused for the adjoint method;
+- StatePrep / QubitStateVector / BasisState are left out and reported;
+  Rot becomes RZ(phi), RY(theta), RZ(omega); an adjoint runs its base with
+  the inverse flag; an operation without a named method on the state-vector
+  class is passed by its matrix, which already holds any adjoint.
+
+Objects are read duck-typed, so PennyLane objects and the stand-ins of
+``circuit.py`` both fit.
 """
 from __future__ import annotations
 
-from typing import List, Tuple
+from dataclasses import dataclass
+from typing import Dict, List, Sequence, Tuple
 
 import numpy as np
 
-pauli_name_map = {"I": "Identity", "X": "PauliX", "Y": "PauliY", "Z": "PauliZ"}
+_LETTER = {"I": "Identity", "X": "PauliX", "Y": "PauliY", "Z": "PauliZ"}
+_NAMED = frozenset(("PauliX", "PauliY", "PauliZ", "Identity", "Hadamard"))
+_PREP = frozenset(("StatePrep", "QubitStateVector", "BasisState"))
+# gates the adjoint sweep takes as products of one-parameter gates:
+# (gate, index of the parameter it takes), in application order
+_PRODUCTS = {"Rot": (("RZ", 0), ("RY", 1), ("RZ", 2))}
 
 
-def _bits(use_csingle: bool) -> str:
-    return "64" if use_csingle else "128"
+@dataclass(frozen=True)
+class _Precision:
+    suffix: str
+    real: type
+    cplx: type
+    index: type
 
 
-def _named_ob_dtype(backend, use_csingle):
-    return getattr(backend, "NamedObsGPU_C" + _bits(use_csingle))
+_PRECISION = {
+    False: _Precision("C128", np.float64, np.complex128, np.int64),
+    True: _Precision("C64", np.float32, np.complex64, np.int32),
+}
 
 
-def _tensor_ob_dtype(backend, use_csingle):
-    return getattr(backend, "TensorProdObsGPU_C" + _bits(use_csingle))
+def _is_product(ob) -> bool:
+    return isinstance(getattr(ob, "name", None), list) and hasattr(ob, "obs")
 
 
-def _hamiltonian_ob_dtype(backend, use_csingle):
-    return getattr(backend, "HamiltonianGPU_C" + _bits(use_csingle)), (np.float32 if use_csingle else np.float64)
+class TapeSerializer:
+    """Serializes tapes for one backend precision (``single``: complex64).
 
+    ``split_sums``: a sum observable gives one row per term (the adjoint
+    method's default) instead of one row for the whole sum.
+    """
 
-def _sparsehamiltonian_ob_dtype(backend, use_csingle):
-    return (getattr(backend, "SparseHamiltonianGPU_C" + _bits(use_csingle)),
-            np.complex64 if use_csingle else np.complex128, np.int32 if use_csingle else np.int64)
+    def __init__(self, backend, wire_map: Dict, single: bool = False, split_sums: bool = True):
+        self.precision = _PRECISION[bool(single)]
+        self.wire_map = wire_map
+        self.split_sums = split_sums
+        suffix = self.precision.suffix
 
+        def cls(stem):
+            return getattr(backend, f"{stem}_{suffix}", None)
 
-def _sv_py_dtype(backend, use_csingle):
-    return getattr(backend, "LightningGPU_C" + _bits(use_csingle))
+        self._named = cls("NamedObsGPU")
+        self._product = cls("TensorProdObsGPU")
+        self._sum = cls("HamiltonianGPU")
+        self._sparse = cls("SparseHamiltonianGPU")
+        self.state_class = cls("LightningGPU")
 
+    def _map(self, wires) -> List[int]:
+        return [self.wire_map[w] for w in wires]
 
-def _is_tensor(ob) -> bool:
-    return hasattr(ob, "obs") and isinstance(getattr(ob, "name", None), list)
+    # ---------------------------------------------------------------- observables
+    def _classify(self, ob) -> str:
+        if _is_product(ob):
+            return "product"
+        name = getattr(ob, "name", None)
+        if name in ("Hamiltonian", "SparseHamiltonian"):
+            return name
+        if name in _NAMED:
+            return "named"
+        if getattr(ob, "_pauli_rep", None) is not None:
+            return "sentence"
+        if name == "Hermitian":
+            raise TypeError("Hermitian observables cannot be differentiated with the adjoint method on this "
+                            "device; express the observable through Pauli words.")
+        raise TypeError(f"Observable {ob} is not supported by the adjoint method; express it through Pauli words.")
 
+    def _named_of(self, name: str, wires: Sequence) -> object:
+        mapped = self._map(wires)
+        return self._named(name, mapped[:1] if name == "Identity" else mapped)
 
-def _serialize_named_ob(o, wires_map: dict, backend, use_csingle: bool):
-    """_serialize.py:142-149."""
-    wires = [wires_map[w] for w in o.wires]
-    if o.name == "Identity":
-        wires = wires[:1]
-    return _named_ob_dtype(backend, use_csingle)(o.name, wires)
+    def _word(self, word: Dict) -> object:
+        """A Pauli word {wire: letter}."""
+        factors = [self._named_of(_LETTER[letter], [wire]) for wire, letter in word.items()]
+        return factors[0] if len(factors) == 1 else self._product(factors)
 
-
-def _serialize_tensor_ob(ob, wires_map: dict, backend, use_csingle: bool):
-    """_serialize.py:152-156."""
-    return _tensor_ob_dtype(backend, use_csingle)(
-        [_serialize_ob(o, wires_map, backend, use_csingle) for o in ob.obs])
-
-
-def _serialize_hamiltonian(ob, wires_map: dict, backend, use_csingle: bool, split_terms: bool = True):
-    """_serialize.py:159-168."""
-    hamiltonian_obs, rtype = _hamiltonian_ob_dtype(backend, use_csingle)
-    coeffs = np.array(ob.coeffs).astype(rtype)
-    terms = [_serialize_ob(t, wires_map, backend, use_csingle) for t in ob.ops]
-    if split_terms:
-        return [hamiltonian_obs(np.array([c], dtype=rtype), [t]) for (c, t) in zip(coeffs, terms)]
-    return hamiltonian_obs(coeffs, terms)
-
-
-def _serialize_sparsehamiltonian(ob, wires_map: dict, backend, use_csingle: bool):
-    """_serialize.py:171-193 (single process)."""
-    sparse_obs, ctype, rtype = _sparsehamiltonian_ob_dtype(backend, use_csingle)
-    spm = ob.sparse_matrix().tocsr()
-    data = np.array(spm.data).astype(ctype)
-    indices = np.array(spm.indices).astype(rtype)
-    offsets = np.array(spm.indptr).astype(rtype)
-    return sparse_obs(data, indices, offsets, [wires_map[w] for w in ob.wires])
-
-
-def _serialize_pauli_word(ob, wires_map: dict, backend, use_csingle: bool):
-    """_serialize.py:204-215: a {wire: 'X'|'Y'|'Z'|'I'} word."""
-    named_obs = _named_ob_dtype(backend, use_csingle)
-    items = list(ob.items())
-    if len(items) == 1:
-        wire, pauli = items[0]
-        return named_obs(pauli_name_map[pauli], [wires_map[wire]])
-    return _tensor_ob_dtype(backend, use_csingle)(
-        [named_obs(pauli_name_map[pauli], [wires_map[wire]]) for wire, pauli in items])
-
-
-def _serialize_pauli_sentence(ob, wires_map: dict, backend, use_csingle: bool, split_terms: bool = True):
-    """_serialize.py:218-229: a {word: coefficient} sentence."""
-    hamiltonian_obs, rtype = _hamiltonian_ob_dtype(backend, use_csingle)
-    pwords, coeffs = zip(*ob.items())
-    terms = [_serialize_pauli_word(pw, wires_map, backend, use_csingle) for pw in pwords]
-    coeffs = np.array(coeffs).astype(rtype)
-    if split_terms:
-        return [hamiltonian_obs(np.array([c], dtype=rtype), [t]) for (c, t) in zip(coeffs, terms)]
-    return hamiltonian_obs(coeffs, terms)
-
-
-def _serialize_ob(ob, wires_map, backend, use_csingle, use_splitting: bool = True):
-    """_serialize.py:232-251."""
-    if _is_tensor(ob):
-        return _serialize_tensor_ob(ob, wires_map, backend, use_csingle)
-    if ob.name == "Hamiltonian":
-        return _serialize_hamiltonian(ob, wires_map, backend, use_csingle, use_splitting)
-    if ob.name == "SparseHamiltonian":
-        return _serialize_sparsehamiltonian(ob, wires_map, backend, use_csingle)
-    if ob.name in ("PauliX", "PauliY", "PauliZ", "Identity", "Hadamard"):
-        return _serialize_named_ob(ob, wires_map, backend, use_csingle)
-    if getattr(ob, "_pauli_rep", None) is not None:
-        return _serialize_pauli_sentence(ob._pauli_rep, wires_map, backend, use_csingle, use_splitting)
-    if ob.name == "Hermitian":
-        raise TypeError("Hermitian observables are not currently supported for adjoint differentiation. "
-                        "Please use Pauli-words only.")
-    raise TypeError(f"Unknown observable found: {ob}. Please use Pauli-words only.")
-
-
-def _serialize_observables(tape, wires_map: dict, backend, use_csingle: bool = False) -> Tuple[List, List[int]]:
-    """_serialize.py:254-279: serialized observables and, per tape
-    observable, the offset of its rows (a split Hamiltonian takes one row per
-    term)."""
-    output = []
-    offsets = [0]
-    for ob in tape.observables:
-        ser_ob = _serialize_ob(ob, wires_map, backend, use_csingle)
-        if isinstance(ser_ob, list):
-            output.extend(ser_ob)
-            offsets.append(offsets[-1] + len(ser_ob))
+    def _sum_terms(self, ob, kind: str) -> Tuple[np.ndarray, List]:
+        """(coefficients, term objects) of a sum observable."""
+        if kind == "Hamiltonian":
+            coeffs, terms = list(ob.coeffs), [self.single(t) for t in ob.ops]
         else:
-            output.append(ser_ob)
-            offsets.append(offsets[-1] + 1)
-    return output, offsets
+            items = list(ob._pauli_rep.items())
+            coeffs, terms = [c for _, c in items], [self._word(w) for w, _ in items]
+        return np.asarray(coeffs).astype(self.precision.real), terms
 
+    def single(self, ob) -> object:
+        """One backend object for ``ob`` (a sum stays whole)."""
+        kind = self._classify(ob)
+        if kind == "product":
+            return self._product([self.single(o) for o in ob.obs])
+        if kind == "named":
+            return self._named_of(ob.name, ob.wires)
+        if kind == "SparseHamiltonian":
+            csr = ob.sparse_matrix().tocsr()
+            p = self.precision
+            return self._sparse(np.asarray(csr.data).astype(p.cplx), np.asarray(csr.indices).astype(p.index),
+                                np.asarray(csr.indptr).astype(p.index), self._map(ob.wires))
+        coeffs, terms = self._sum_terms(ob, kind)
+        return self._sum(coeffs, terms)
 
-def _serialize_ops(tape, wires_map: dict, backend, use_csingle: bool = False):
-    """_serialize.py:282-336: ((names, params, wires, inverses, matrices),
-    uses_stateprep)."""
-    names, params, wires, inverses, mats = [], [], [], [], []
-    uses_stateprep = False
-    sv_py = _sv_py_dtype(backend, use_csingle)
-    rtype = np.float32 if use_csingle else np.float64
-    ctype = np.complex64 if use_csingle else np.complex128
-    for o in tape.operations:
-        if o.name in ("StatePrep", "QubitStateVector", "BasisState"):
-            uses_stateprep = True
-            continue
-        op_list = o.expand().operations if o.name == "Rot" else [o]
-        for single_op in op_list:
-            is_inverse = hasattr(single_op, "base")
-            name = single_op.base.name if is_inverse else single_op.name
-            names.append(name)
-            if getattr(sv_py, name, None) is None:
-                params.append(np.zeros(0, dtype=rtype))
-                mats.append(np.asarray(single_op.matrix(), dtype=ctype).ravel(order="C"))
-                is_inverse = False  # the matrix already holds the adjoint
+    def rows(self, ob) -> List:
+        """The backend objects standing for ``ob``, one per Jacobian row."""
+        kind = self._classify(ob)
+        if self.split_sums and kind in ("Hamiltonian", "sentence"):
+            coeffs, terms = self._sum_terms(ob, kind)
+            return [self._sum(coeffs[i:i + 1], [t]) for i, t in enumerate(terms)]
+        return [self.single(ob)]
+
+    def observables(self, tape) -> Tuple[List, List[int]]:
+        """All rows of the tape's observables and the row offsets: the rows
+        of observable k are rows[offsets[k]:offsets[k + 1]]."""
+        per_ob = [self.rows(ob) for ob in tape.observables]
+        offsets = [0]
+        for r in per_ob:
+            offsets.append(offsets[-1] + len(r))
+        return [x for r in per_ob for x in r], offsets
+
+    # ---------------------------------------------------------------- operations
+    def _records(self, op) -> List[Tuple]:
+        """(name, parameters, wires, inverse, matrix or None) per backend gate."""
+        wires = self._map(op.wires)
+        if op.name in _PRODUCTS:
+            par = list(op.parameters)
+            return [(g, [par[k]], wires, False, None) for g, k in _PRODUCTS[op.name]]
+        base = getattr(op, "base", None)
+        name = base.name if base is not None else op.name
+        if getattr(self.state_class, name, None) is not None:
+            return [(name, list(op.parameters), wires, base is not None, None)]
+        return [(name, [], wires, False, np.asarray(op.matrix()))]
+
+    def operations(self, tape):
+        """((names, params, wires, inverses, matrices), holds_state_preparation)."""
+        p = self.precision
+        records, prep = [], False
+        for op in tape.operations:
+            if op.name in _PREP:
+                prep = True
             else:
-                params.append(np.asarray(single_op.parameters, dtype=rtype))
-                mats.append(np.zeros(0, dtype=ctype))
-            wires.append([wires_map[w] for w in single_op.wires])
-            inverses.append(is_inverse)
-    return (names, params, wires, inverses, mats), uses_stateprep
+                records.extend(self._records(op))
+        lists = ([], [], [], [], [])
+        for name, par, wires, inverse, mat in records:
+            lists[0].append(name)
+            lists[1].append(np.asarray(par, dtype=p.real))
+            lists[2].append(wires)
+            lists[3].append(inverse)
+            lists[4].append(np.zeros(0, dtype=p.cplx) if mat is None else np.asarray(mat, dtype=p.cplx).ravel(order="C"))
+        return lists, prep

@@ -41,6 +41,9 @@ from . import _serialize as ser
 
 _name_map = {"PauliX": "X", "PauliY": "Y", "PauliZ": "Z", "Identity": "I"}
 _STATE_PREP = ("StatePrep", "QubitStateVector")
+_PREP_ALL = _STATE_PREP + ("BasisState",)
+_SHOTS_WARNING = ("Requested adjoint differentiation to be computed with finite shots. The derivative is always "
+                  "exact when using the adjoint differentiation method.")
 
 
 class DeviceError(Exception):
@@ -93,7 +96,8 @@ class LightningGPU:
         self.shots = shots
         self._batch_obs = batch_obs
         self._backend = _backend if _backend is not None else _load_backend()
-        self._gpu_state = ser._sv_py_dtype(self._backend, self.use_csingle)(self.num_wires)
+        self._gpu_state = getattr(self._backend, "LightningGPU_C" + ("64" if self.use_csingle else "128"))(
+            self.num_wires)
         if seed is not None:
             self._gpu_state.setSeed(int(seed))
         self._samples = None
@@ -340,13 +344,40 @@ class LightningGPU:
     def _adj(self):
         return getattr(self._backend, "AdjointJacobianGPU_C" + ("64" if self.use_csingle else "128"))()
 
+    def _serializer(self) -> ser.TapeSerializer:
+        return ser.TapeSerializer(self._backend, self.wire_map, self.use_csingle)
+
+    @staticmethod
+    def _parameter_columns(tape, shift: int):
+        """Which tape parameters the backend differentiates: for every
+        trainable parameter that is not a state preparation's, its index
+        among the parameters the backend sees (the serialized operations
+        leave the state preparation's parameter out, ``shift``) and its
+        column in the device's Jacobian (lightning_gpu.py:675-693)."""
+        backend_index, column = [], []
+        for col, par in enumerate(sorted(tape.trainable_params)):
+            if tape.get_operation(col)[0].name in _PREP_ALL:
+                continue
+            backend_index.append(par - shift)
+            column.append(col)
+        return backend_index, column
+
+    def _observable_batches(self, count: int):
+        """Slices of the serialized observables run per backend call: all of
+        them at once, or groups of ``batch_obs`` (lightning_gpu.py:700-721)."""
+        if not self._batch_obs or count == 0:
+            return None
+        size = count if isinstance(self._batch_obs, bool) else max(1, int(self._batch_obs))
+        return [slice(lo, min(lo + size, count)) for lo in range(0, count, size)]
+
     def adjoint_jacobian(self, tape, starting_state=None, use_device_state=False, **kwargs):
-        """lightning_gpu.py:639-758."""
+        """The Jacobian of the tape's expectation values by the adjoint
+        method (lightning_gpu.py:639-758): one row per tape observable, one
+        column per trainable parameter; a split sum's rows are added back."""
         if self.shots is not None:
-            warnings.warn("Requested adjoint differentiation to be computed with finite shots. The derivative is "
-                          "always exact when using the adjoint differentiation method.", UserWarning)
+            warnings.warn(_SHOTS_WARNING, UserWarning)
         self._check_adjdiff_supported_measurements(tape.measurements)
-        if len(tape.trainable_params) == 0:
+        if not tape.trainable_params:
             return np.array(0)
         self._check_adjdiff_supported_operations(tape.operations)
         if starting_state is not None:
@@ -355,75 +386,62 @@ class LightningGPU:
             self.reset()
             self.apply(tape.operations)
 
+        serializer = self._serializer()
+        rows, offsets = serializer.observables(tape)
+        op_lists, has_prep = serializer.operations(tape)
         adj = self._adj()
-        obs_serialized, obs_offsets = ser._serialize_observables(tape, self.wire_map, self._backend,
-                                                                 self.use_csingle)
-        ops_serialized, use_sp = ser._serialize_ops(tape, self.wire_map, self._backend, self.use_csingle)
-        ops_serialized = adj.create_ops_list(*ops_serialized)
+        ops = adj.create_ops_list(*op_lists)
+        params, columns = self._parameter_columns(tape, 1 if has_prep else 0)
 
-        tp_shift, record_tp_rows = [], []
-        all_params = 0
-        for op_idx, tp in enumerate(sorted(tape.trainable_params)):
-            op, _, _ = tape.get_operation(op_idx)
-            if op.name not in _STATE_PREP and op.name != "BasisState":
-                tp_shift.append(tp)
-                record_tp_rows.append(all_params)
-            all_params += 1
-        if use_sp:
-            tp_shift = [i - 1 for i in tp_shift]
-
-        if self._batch_obs:
-            num_obs = len(obs_serialized)
-            batch = num_obs if isinstance(self._batch_obs, bool) else int(self._batch_obs)
-            jac = []
-            for chunk in range(0, num_obs, batch):
-                jac.extend(adj.adjoint_jacobian_batched(self._gpu_state, obs_serialized[chunk:chunk + batch],
-                                                        ops_serialized, tp_shift))
+        batches = self._observable_batches(len(rows))
+        if batches is None:
+            raw = adj.adjoint_jacobian(self._gpu_state, rows, ops, params)
         else:
-            jac = adj.adjoint_jacobian(self._gpu_state, obs_serialized, ops_serialized, tp_shift)
-
-        jac = np.array(jac, dtype=np.float64).reshape(-1, len(tp_shift))
-        jac_r = np.zeros((len(tape.observables), all_params))
-        for idx in range(len(obs_offsets) - 1):
-            rows = jac[obs_offsets[idx]:obs_offsets[idx + 1], :]
-            jac_r[idx, record_tp_rows] = np.sum(rows, axis=0)
-        return self._adjoint_jacobian_processing(jac_r)
+            raw = [r for b in batches for r in adj.adjoint_jacobian_batched(self._gpu_state, rows[b], ops, params)]
+        per_row = np.asarray(raw, dtype=np.float64).reshape(len(rows), len(params))
+        # rows of one tape observable (a split sum has several) add up
+        per_ob = np.stack([per_row[lo:hi].sum(axis=0) for lo, hi in zip(offsets[:-1], offsets[1:])]) \
+            if len(offsets) > 1 else np.zeros((0, len(params)))
+        full = np.zeros((len(offsets) - 1, len(tape.trainable_params)))
+        full[:, columns] = per_ob
+        return self._adjoint_jacobian_processing(full)
 
     @staticmethod
     def _adjoint_jacobian_processing(jac):
-        """lightning_gpu.py:755-768."""
+        """The return layout of the reference (lightning_gpu.py:755-768): the
+        squeezed Jacobian as a 0-d array, or as (nested) tuples of 0-d
+        arrays."""
         jac = np.squeeze(jac)
-        if jac.ndim == 0:
-            return np.array(jac)
-        if jac.ndim == 1:
-            return tuple(np.array(j) for j in jac)
-        return tuple(tuple(np.array(j_) for j_ in j) for j in jac)
+
+        def nest(a):
+            return np.array(a) if a.ndim == 0 else tuple(nest(x) for x in a)
+
+        return nest(jac)
 
     def vjp(self, measurements, dy, starting_state=None, use_device_state=False):
-        """lightning_gpu.py:771-810: a function of the tape returning
-        dy . J, computed as the Jacobian of <sum_j dy_j O_j>."""
+        """Vector-Jacobian product dy . J (lightning_gpu.py:771-810), returned
+        as a function of the tape: the Jacobian of the single expectation
+        <sum_j dy_j O_j>, one adjoint sweep instead of one per observable."""
         from .circuit import Hamiltonian, MeasurementProcess
 
         if self.shots is not None:
-            warnings.warn("Requested adjoint differentiation to be computed with finite shots. The derivative is "
-                          "always exact when using the adjoint differentiation method.", UserWarning)
-        rt = self._check_adjdiff_supported_measurements(measurements)
+            warnings.warn(_SHOTS_WARNING, UserWarning)
+        kind = self._check_adjdiff_supported_measurements(measurements)
         dy = np.asarray(dy)
-        if np.allclose(dy, 0) or rt is None:
+        if kind is None or not np.any(dy):
             return lambda tape: np.zeros(len(tape.trainable_params))
-        if len(dy) != len(measurements):
-            raise ValueError("Number of observables in the tape must be the same as the length of dy in the vjp "
-                             "method")
+        if dy.shape[0] != len(measurements):
+            raise ValueError(f"dy has {dy.shape[0]} entries for {len(measurements)} measured observables; the "
+                             "vjp needs one per observable")
         if np.iscomplexobj(dy):
-            raise ValueError("The vjp method only works with a real-valued dy when the tape is returning an "
-                             "expectation value")
-        ham = Hamiltonian(dy, [m.obs for m in measurements])
+            raise ValueError("the vjp of expectation values takes a real dy")
+        weighted = Hamiltonian(dy, [m.obs for m in measurements])
 
         def processing_fn(tape):
-            if len(tape.trainable_params) == 0:
+            if not tape.trainable_params:
                 return np.array([], dtype=self.C_DTYPE)
-            new_tape = tape.copy()
-            new_tape.measurements = [MeasurementProcess("expval", ham)]
-            return self.adjoint_jacobian(new_tape, starting_state, use_device_state)
+            reweighted = tape.copy()
+            reweighted.measurements = [MeasurementProcess("expval", weighted)]
+            return self.adjoint_jacobian(reweighted, starting_state, use_device_state)
 
         return processing_fn

@@ -141,7 +141,7 @@ def test_serialize_ops_rot_adjoint_matrix_and_stateprep():
            qc.Adjoint(qc.Operation("RX", [0], [0.4])), qc.Operation("QubitUnitary", [2], [], matrix=U),
            qc.Adjoint(qc.Operation("QubitUnitary", [2], [], matrix=U))]
     tape = qc.QuantumTape(ops, [qc.expval(qc.PauliZ(0))])
-    (names, params, wires, inv, mats), use_sp = ser._serialize_ops(tape, {0: 0, 1: 1, 2: 2}, FakeBackend, False)
+    (names, params, wires, inv, mats), use_sp = ser.TapeSerializer(FakeBackend, {0: 0, 1: 1, 2: 2}).operations(tape)
     assert use_sp
     assert names == ["RZ", "RY", "RZ", "RX", "QubitUnitary", "QubitUnitary"]
     assert [list(p) for p in params] == [[0.1], [0.2], [0.3], [0.4], [], []]
@@ -154,19 +154,19 @@ def test_serialize_ops_rot_adjoint_matrix_and_stateprep():
 def test_serialize_observables_offsets_split_and_identity():
     H = qc.Hamiltonian([1.0, 2.0, 3.0], [qc.PauliX(0), qc.PauliZ(1) @ qc.PauliY(2), qc.Identity(1)])
     tape = qc.QuantumTape([], [qc.expval(qc.PauliZ(0)), qc.expval(H), qc.expval(qc.Identity([1, 2]))])
-    obs, offsets = ser._serialize_observables(tape, {0: 0, 1: 1, 2: 2}, FakeBackend, False)
+    obs, offsets = ser.TapeSerializer(FakeBackend, {0: 0, 1: 1, 2: 2}).observables(tape)
     assert offsets == [0, 1, 4, 5]
     assert obs[0].kind == "Named" and obs[0].args == ("PauliZ", [0])
     assert [o.kind for o in obs[1:4]] == ["Ham"] * 3
     assert list(obs[2].args[0]) == [2.0] and obs[2].args[1][0].kind == "Tensor"
     assert obs[4].args == ("Identity", [1])  # Identity keeps its first wire only
     with pytest.raises(TypeError, match="Hermitian"):
-        ser._serialize_ob(qc.Hermitian(np.eye(2), [0]), {0: 0}, FakeBackend, False)
+        ser.TapeSerializer(FakeBackend, {0: 0}).rows(qc.Hermitian(np.eye(2), [0]))
 
 
 def test_serialize_sparse_hamiltonian_csr_arrays():
     H = sp.random(8, 8, density=0.4, random_state=3, format="csr") + 0j
-    o = ser._serialize_ob(qc.SparseHamiltonian(H, [0, 1, 2]), {0: 0, 1: 1, 2: 2}, FakeBackend, False)
+    o = ser.TapeSerializer(FakeBackend, {0: 0, 1: 1, 2: 2}).single(qc.SparseHamiltonian(H, [0, 1, 2]))
     data, idx, ptr, wires = o.args
     assert data.dtype == np.complex128 and idx.dtype == np.int64 and ptr.dtype == np.int64
     np.testing.assert_array_equal(ptr, H.indptr)
