@@ -219,6 +219,19 @@ lgh_status lgh_expval_matrix(const lgh_sv *sv, const int32_t *wires, int32_t num
 lgh_status lgh_expval_pauli_words(const lgh_sv *sv, int64_t num_words, const char *paulis,
                                   const int32_t *wires, const int64_t *offsets,
                                   const double *coeffs, double *per_word, double *result);
+/* lgh_expval_pauli_words without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results (bit for bit) as lgh_expval_pauli_words; at most 1024
+ * words, and the reduction partials of a pass plus 16 B per word must fit a
+ * fixed 512 KiB device scratch array (16 B * 16 * 8 * CUs + 16 B * words;
+ * 256 KiB of partials on a 256-CU device), else LGH_ERR_INVALID_ARGUMENT.
+ * The scratch array is shared by every call on a device: calls that overlap
+ * on one device race and must be serialised by the caller.  The first call
+ * on a device looks up the array's address; later calls allocate nothing. */
+lgh_status lgh_expval_pauli_words_noalloc(const lgh_sv *sv, int64_t num_words, const char *paulis,
+                                          const int32_t *wires, const int64_t *offsets, const double *coeffs,
+                                          double *per_word, double *result);
 /* CSR kept resident on the device (replaces the per-call upload of
  * getExpectationValueOnSparseSpMV, SVCM:795-922).  values: complex128. */
 lgh_status lgh_csr_create(lgh_csr **out, int device, lgh_dtype dtype, int64_t num_rows,

@@ -612,6 +612,26 @@ template <class R> void expval_t(const lgh_sv *sv, const int32_t *wires, int k, 
     *im = out[1];
 }
 
+// <P> = i^{nY} (re + i im) for the word's accumulated (re, im): its real part
+double word_value(double re, double im, int ny) {
+    switch (ny & 3) {
+    case 0: return re;
+    case 1: return -im;
+    case 2: return -re;
+    default: return im;
+    }
+}
+
+// sum_w Re(c_w <P_w>) with <P_w> real, in word order; per_word (nullable) gets <P_w>
+double weighted_total(int64_t nwords, const double *coeffs, const double *vals, double *per_word) {
+    double tot = 0.0;
+    for (int64_t w = 0; w < nwords; ++w) {
+        tot += coeffs[2 * w] * vals[w];
+        if (per_word) per_word[w] = vals[w];
+    }
+    return tot;
+}
+
 template <class R>
 void pauli_t(const lgh_sv *sv, int64_t nwords, const char *paulis, const int32_t *wires, const int64_t *offsets,
              const double *coeffs, double *per_word, double *result) {
@@ -670,26 +690,129 @@ void pauli_t(const lgh_sv *sv, int64_t nwords, const char *paulis, const int32_t
     }
     std::vector<double> out(2 * size_t(nwords));
     if (nwords > 0) fetch(ctx, results, int(2 * nwords), out.data());
-    for (size_t q = 0; q < slot_word.size(); ++q) {
-        const W &x = words[size_t(slot_word[q])];
-        // <P> = i^{nY} (re + i im): real part
-        const double re = out[2 * q], im = out[2 * q + 1];
-        double v;
-        switch (x.ny & 3) {
-        case 0: v = re; break;
-        case 1: v = -im; break;
-        case 2: v = -re; break;
-        default: v = im; break;
-        }
-        vals[size_t(slot_word[q])] = v;
+    for (size_t q = 0; q < slot_word.size(); ++q)
+        vals[size_t(slot_word[q])] = word_value(out[2 * q], out[2 * q + 1], words[size_t(slot_word[q])].ny);
+    *result = weighted_total(nwords, coeffs, vals.data(), per_word);
+}
+
+// ---------------------------------------------------------------- Pauli words without allocation
+//
+// lgh_expval_pauli_words_noalloc runs the passes of pauli_t (same kernels,
+// same grouping by flip mask, same chunking, so the same results bit for bit)
+// with its reduction partials and word results in `scratch`, a device array
+// of fixed size that every call on one device shares.  The host side keeps
+// its bookkeeping in fixed-size arrays on the stack.  Nothing is allocated
+// after the first call on a device, which looks up the array's device address
+// (and the device's CU count used to size the passes).
+//
+// Layout of `scratch` for one call: partials of one pass (2 * 8 words *
+// blocks doubles) | (re, im) of every word.  Every entry a call reads was
+// written earlier in the same call (each pass writes all its partials, the
+// reduction writes the word's results), so nothing needs clearing; two calls
+// that overlap on one device race on the array.
+
+constexpr size_t kNoAllocScratchBytes = size_t(512) << 10;
+constexpr int64_t kNoAllocMaxWords = 1024;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (the current device), looked up once
+// per device.  The array's host-side name is not a device address.
+void *scratch_on(int device) {
+    LGH_REQUIRE(device >= 0 && device < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        void *p = nullptr;
+        LGH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        LGH_REQUIRE(p != nullptr, "scratch array has no device address");
+        (void)sm_count(device); // cached from here on: later calls only read it
+        g_scratch_dev[device] = p;
     }
-    double tot = 0.0;
+    return g_scratch_dev[device];
+}
+
+template <class R>
+void pauli_noalloc_t(const lgh_sv *sv, int64_t nwords, const char *paulis, const int32_t *wires,
+                     const int64_t *offsets, const double *coeffs, double *per_word, double *result) {
+    LGH_REQUIRE(nwords <= kNoAllocMaxWords, "lgh_expval_pauli_words_noalloc takes at most " +
+                                                std::to_string(kNoAllocMaxWords) + " words (got " +
+                                                std::to_string(nwords) + ")");
+    StreamCtx *ctx = sv->ctx;
+    const int n = sv->n;
+    char *base = static_cast<char *>(scratch_on(sv->device));
+    uint64_t flip[kNoAllocMaxWords], zy[kNoAllocMaxWords];
+    int32_t ny[kNoAllocMaxWords], slot_word[kNoAllocMaxWords];
+    bool done[kNoAllocMaxWords];
     for (int64_t w = 0; w < nwords; ++w) {
-        // Re(c_w * <P_w>) with <P_w> real
-        tot += coeffs[2 * w] * vals[size_t(w)];
-        if (per_word) per_word[w] = vals[size_t(w)];
+        flip[w] = zy[w] = 0;
+        ny[w] = 0;
+        done[w] = false;
+        for (int64_t j = offsets[w]; j < offsets[w + 1]; ++j) {
+            const int wire = wires[j];
+            LGH_REQUIRE(wire >= 0 && wire < n, "Pauli word wire out of range");
+            const uint64_t bit = 1ull << (n - 1 - wire);
+            switch (paulis[j]) {
+            case 'I': break;
+            case 'X': flip[w] |= bit; break;
+            case 'Y': flip[w] |= bit; zy[w] |= bit; ny[w]++; break;
+            case 'Z': zy[w] |= bit; break;
+            default: LGH_REQUIRE(false, "unknown Pauli factor (words use I, X, Y, Z)");
+            }
+        }
     }
-    *result = tot;
+    const uint64_t len = sv->length();
+    const unsigned max_blocks = std::max(chunks_for(len, sv->device).blocks, chunks_for(len / 2, sv->device).blocks);
+    const size_t part_doubles = size_t(2 * kMaxWordsPerPass) * max_blocks;
+    const size_t need = sizeof(double) * (part_doubles + 2 * size_t(nwords));
+    LGH_REQUIRE(need <= kNoAllocScratchBytes, "lgh_expval_pauli_words_noalloc: " + std::to_string(need) +
+                                                  " bytes of scratch needed, " +
+                                                  std::to_string(kNoAllocScratchBytes) + " available");
+    double *partials = reinterpret_cast<double *>(base);
+    double *results = partials + part_doubles;
+    // groups in ascending flip mask, words of a group in ascending order: the
+    // passes of pauli_t
+    int64_t nslots = 0;
+    for (;;) {
+        bool any = false;
+        uint64_t f = 0;
+        for (int64_t w = 0; w < nwords; ++w)
+            if (!done[w] && (!any || flip[w] < f)) {
+                f = flip[w];
+                any = true;
+            }
+        if (!any) break;
+        const int pivot = f ? __builtin_ctzll(f) : -1;
+        const uint64_t count = f ? len / 2 : len;
+        const Chunks ch = chunks_for(count, sv->device);
+        WordSet ws{};
+        auto launch = [&] {
+            hipLaunchKernelGGL(k_pauli<R>, dim3(ch.blocks), dim3(kBlock), 0, ctx->stream, cdata<R>(sv), count,
+                               ch.chunk, pivot, f, ws, partials);
+            LGH_LAUNCH_CHECK();
+            launch_reduce(ctx, partials, ch.blocks, 2 * ws.count, results + 2 * (nslots - ws.count));
+            ws.count = 0;
+        };
+        for (int64_t w = 0; w < nwords; ++w) {
+            if (done[w] || flip[w] != f) continue;
+            done[w] = true;
+            ws.zy[ws.count++] = zy[w];
+            slot_word[nslots++] = int32_t(w);
+            if (ws.count == kMaxWordsPerPass) launch();
+        }
+        if (ws.count > 0) launch();
+    }
+    double out[2 * kNoAllocMaxWords];
+    double vals[kNoAllocMaxWords];
+    if (nwords > 0) {
+        LGH_HIP(hipMemcpyAsync(out, results, sizeof(double) * 2 * size_t(nwords), hipMemcpyDeviceToHost, ctx->stream));
+        ctx->sync();
+    }
+    for (int64_t q = 0; q < nslots; ++q) vals[slot_word[q]] = word_value(out[2 * q], out[2 * q + 1], ny[slot_word[q]]);
+    *result = weighted_total(nwords, coeffs, vals, per_word);
 }
 
 template <class I, class R>
@@ -922,6 +1045,23 @@ lgh_status lgh_expval_pauli_words(const lgh_sv *sv, int64_t num_words, const cha
         DeviceGuard g(sv->device);
         by_dtype(sv, [&](auto r) {
             pauli_t<decltype(r)>(sv, num_words, paulis, wires, offsets, coeffs, per_word, result);
+        });
+    });
+}
+
+lgh_status lgh_expval_pauli_words_noalloc(const lgh_sv *sv, int64_t num_words, const char *paulis,
+                                          const int32_t *wires, const int64_t *offsets, const double *coeffs,
+                                          double *per_word, double *result) {
+    return guard([&] {
+        LGH_REQUIRE(sv && result, "null pointer");
+        LGH_REQUIRE(num_words >= 0, "negative word count");
+        LGH_REQUIRE(num_words == 0 || (paulis && offsets && coeffs), "null pointer");
+        for (int64_t w = 0; w < num_words; ++w)
+            LGH_REQUIRE(offsets[w + 1] >= offsets[w] && (offsets[w + 1] == offsets[w] || wires),
+                        "bad Pauli word offsets");
+        DeviceGuard g(sv->device);
+        by_dtype(sv, [&](auto r) {
+            pauli_noalloc_t<decltype(r)>(sv, num_words, paulis, wires, offsets, coeffs, per_word, result);
         });
     });
 }

@@ -92,6 +92,8 @@ def lib():
         L.lgh_scale.argtypes = [vp, C.c_double, C.c_double]
         L.lgh_expval_matrix.argtypes = [vp, vp, C.c_int32, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.lgh_expval_pauli_words.argtypes = [vp, C.c_int64, C.c_char_p, vp, vp, vp, vp, C.POINTER(C.c_double)]
+        L.lgh_expval_pauli_words_noalloc.argtypes = [vp, C.c_int64, C.c_char_p, vp, vp, vp, vp,
+                                                     C.POINTER(C.c_double)]
         L.lgh_csr_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int64, vp, vp, vp, C.c_int64]
         L.lgh_csr_destroy.argtypes = [vp]
         L.lgh_expval_csr.argtypes = [vp, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
@@ -322,7 +324,10 @@ class State:
                                       C.byref(im)))
         return complex(re.value, im.value)
 
-    def expval_pauli_words(self, words, wires, coeffs):
+    def expval_pauli_words(self, words, wires, coeffs, noalloc=False):
+        """sum_w Re(c_w <P_w>) and the per-word <P_w>.  noalloc: through
+        lgh_expval_pauli_words_noalloc (fixed device scratch, at most 1024
+        words, no allocation after the first call on a device)."""
         chars = "".join(words).encode()
         wf = np.ascontiguousarray([x for ws in wires for x in ws] or [0], dtype=np.int32)
         offs = np.zeros(len(words) + 1, dtype=np.int64)
@@ -330,8 +335,9 @@ class State:
         c = np.ascontiguousarray(np.asarray(coeffs, dtype=np.complex128))
         per = np.zeros(max(1, len(words)))
         tot = C.c_double()
-        check(lib().lgh_expval_pauli_words(self.handle, len(words), chars, wf.ctypes.data, offs.ctypes.data,
-                                           c.ctypes.data, per.ctypes.data, C.byref(tot)))
+        fn = lib().lgh_expval_pauli_words_noalloc if noalloc else lib().lgh_expval_pauli_words
+        check(fn(self.handle, len(words), chars, wf.ctypes.data, offs.ctypes.data, c.ctypes.data, per.ctypes.data,
+                 C.byref(tot)))
         return tot.value, per[: len(words)]
 
     def probs(self, wires) -> np.ndarray:
