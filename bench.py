@@ -57,6 +57,14 @@ def parse():
     ap.add_argument("--serve-arrivals", type=int, default=400)
     ap.add_argument("--serve-rate", type=float, default=2000.0)
     ap.add_argument("--serve-streams", type=int, default=4)
+    ap.add_argument("--serve-batch-max", type=int, default=16,
+                    help="circuits of at most this many qubits are batched; larger ones run on their own states")
+    ap.add_argument("--serve-plain-sample", type=int, default=10,
+                    help="requests per (qubits, gradient) class timed in the plain per-circuit form")
+    ap.add_argument("--serve-sweep", type=float, nargs="*", default=None, metavar="RATE",
+                    help="offered rates (requests/s) of a sustainable-rate sweep")
+    ap.add_argument("--serve-sweep-arrivals", type=int, default=600)
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-serial-sample", type=int, default=256,
@@ -260,23 +268,23 @@ def bench_measurements(N, args):
     return out
 
 
-def bench_serving(N, args):
-    """Serving workload: Poisson arrivals (seeded exponential gaps) at
-    ``--serve-rate`` per second; n from {10, 14, 18, 22, 26, 30} with weights
-    8:6:4:3:2:1; layers ~ U{2..8} of [RY, RZ on every wire; CZ on even then odd
-    pairs]; complex128; half the requests ask for <Z_0>, half for its gradient.
-    A fixed number of arrivals is submitted on schedule and drained."""
-    import threading
-
+def serving_requests(args, arrivals, rate, seed, max_qubits=None):
+    """Config-5 arrivals: Poisson at ``rate`` per second (seeded exponential
+    gaps); n from {10, 14, 18, 22, 26, 30} with weights 8:6:4:3:2:1 (classes
+    above ``max_qubits`` dropped); layers ~ U{2..8} of [RY, RZ on every wire;
+    CZ on even then odd pairs]; half the requests ask for <Z_0>, half for its
+    gradient."""
     import numpy as np
 
-    rng = np.random.default_rng(args.seed)
+    rng = np.random.default_rng(seed)
     sizes = np.array([10, 14, 18, 22, 26, 30])
     weights = np.array([8, 6, 4, 3, 2, 1], dtype=float)
+    if max_qubits is not None:
+        weights = np.where(sizes <= max_qubits, weights, 0.0)
     reqs = []
     t = 0.0
-    for i in range(args.serve_arrivals):
-        t += rng.exponential(1.0 / args.serve_rate)
+    for i in range(arrivals):
+        t += rng.exponential(1.0 / rate)
         n = int(rng.choice(sizes, p=weights / weights.sum()))
         layers = int(rng.integers(2, 9))
         ops = []
@@ -288,11 +296,19 @@ def bench_serving(N, args):
                 for w in range(s0, n - 1, 2):
                     ops.append(("CZ", [w, w + 1], [], False))
         reqs.append((t, n, ops, bool(i % 2)))
-    srv = N.Server(num_streams=args.serve_streams)
+    return reqs
+
+
+def serve_stream(N, reqs, streams, batch_max):
+    """Submits ``reqs`` on their schedule to a fresh server and drains it.
+    Returns throughput and latency figures (queueing counted from the
+    scheduled arrival)."""
+    import numpy as np
+
+    srv = N.Server(num_streams=streams, batch_max_qubits=batch_max)
     # warm the template cache and the allocator with one request per class
-    for n in sizes:
+    for n in sorted({r[1] for r in reqs}):
         srv.wait(srv.submit(int(n), [("RY", [0], [0.1], False)], [("Z", 0)], True))
-    log(f"[bench] serving: {len(reqs)} arrivals at {args.serve_rate}/s over {reqs[-1][0]:.2f} s")
     tickets = [None] * len(reqs)
     late = np.zeros(len(reqs))  # submission after the scheduled arrival (host side), ms
     t0 = time.perf_counter()
@@ -306,7 +322,6 @@ def bench_serving(N, args):
     lat = {}
     for i, ((ta, n, ops, grad), tk) in enumerate(zip(reqs, tickets)):
         _, _, tm = srv.wait(tk)
-        # queueing counted from the scheduled arrival
         lat.setdefault(int(n), []).append((tm[0] + late[i], tm[1], tm[2] + late[i]))
     t_all = time.perf_counter() - t0
     st = srv.stats()
@@ -322,10 +337,11 @@ def bench_serving(N, args):
         alltm.append(a)
         per[str(n)] = {"count": len(a), "latency_ms_p50": pct(a[:, 2], 50), "latency_ms_p99": pct(a[:, 2], 99),
                        "queue_ms_p50": pct(a[:, 0], 50), "queue_ms_p99": pct(a[:, 0], 99),
-                       "run_ms_p50": pct(a[:, 1], 50)}
+                       "run_ms_p50": pct(a[:, 1], 50), "run_ms_p99": pct(a[:, 1], 99)}
     a = np.concatenate(alltm)
+    offered = len(reqs) / reqs[-1][0] if reqs[-1][0] > 0 else float("inf")
     return {
-        "arrivals": len(reqs), "offered_rate": args.serve_rate, "streams": args.serve_streams,
+        "arrivals": len(reqs), "offered_rate": round(offered, 1), "streams": streams, "batch_max_qubits": batch_max,
         "submit_window_s": round(t_sub, 3), "drain_s": round(t_all, 3),
         "submit_lateness_ms_p99": pct(late, 99),
         "achieved_circuits_per_s": round(len(reqs) / t_all, 1),
@@ -333,6 +349,76 @@ def bench_serving(N, args):
         "queue_ms_p50": pct(a[:, 0], 50), "queue_ms_p99": pct(a[:, 0], 99),
         "per_class": per, "engine": st,
     }
+
+
+def per_circuit_times(N, reqs):
+    """The plain per-circuit form: requests one after another, each on its
+    own states through the single-circuit entry points (one stream, no
+    batching, no overlap).  Mean wall seconds per request by (qubits,
+    gradient) class."""
+    import numpy as np
+
+    srv = N.Server(num_streams=1, batch_max_qubits=0)
+    for n in sorted({r[1] for r in reqs}):
+        srv.wait(srv.submit(int(n), [("RY", [0], [0.1], False)], [("Z", 0)], True))
+    per = {}
+    for _, n, ops, grad in reqs:
+        t0 = time.perf_counter()
+        srv.wait(srv.submit(n, ops, [("Z", 0)], grad))
+        per.setdefault((n, grad), []).append(time.perf_counter() - t0)
+    srv.close()
+    return {k: float(np.mean(v)) for k, v in per.items()}
+
+
+def plain_rate(reqs, times):
+    """Circuits per second of the per-circuit form over ``reqs``, from the
+    per-class mean times (every class of ``reqs`` must have been timed)."""
+    return len(reqs) / sum(times[(n, grad)] for _, n, _, grad in reqs)
+
+
+def bench_serving(N, args):
+    """Serving workload (config 5): ``--serve-arrivals`` Poisson arrivals at
+    ``--serve-rate`` per second submitted on schedule and drained; the same
+    stream restricted to the classes up to 22 qubits; the plain per-circuit
+    form timed on a sample of each stream for the served / per-circuit
+    circuits-per-second ratio; with ``--serve-sweep``, a sweep of offered
+    rates for the sustainable rate."""
+    out = {}
+    for label, maxq in (("mix_10_30", None), ("mix_10_22", 22)):
+        reqs = serving_requests(args, args.serve_arrivals, args.serve_rate, args.seed, maxq)
+        log(f"[bench] serving {label}: {len(reqs)} arrivals at {args.serve_rate}/s over {reqs[-1][0]:.2f} s")
+        r = serve_stream(N, reqs, args.serve_streams, args.serve_batch_max)
+        # capacity: the same requests all submitted at once
+        sat = serve_stream(N, [(0.0,) + q[1:] for q in reqs], args.serve_streams, args.serve_batch_max)
+        # per-circuit form: every (qubits, gradient) class timed on a sample
+        sample, seen = [], {}
+        for q in reqs:
+            key = (q[1], q[3])
+            if seen.get(key, 0) < args.serve_plain_sample:
+                seen[key] = seen.get(key, 0) + 1
+                sample.append(q)
+        times = per_circuit_times(N, sample)
+        plain = plain_rate(reqs, times)
+        log(f"[bench] serving {label}: {r['achieved_circuits_per_s']} circuits/s at the offered rate, "
+            f"{sat['achieved_circuits_per_s']} at saturation, {plain:.1f} per-circuit form")
+        r["saturation"] = {k: sat[k] for k in ("achieved_circuits_per_s", "drain_s", "latency_ms_p50",
+                                               "latency_ms_p99", "engine")}
+        r["per_circuit_form"] = {"circuits_per_s": round(plain, 1), "sample_per_class": args.serve_plain_sample,
+                                 "mean_ms_by_class": {f"{n}q{'_grad' if g else ''}": round(t * 1e3, 3)
+                                                      for (n, g), t in sorted(times.items())}}
+        r["served_vs_per_circuit"] = round(sat["achieved_circuits_per_s"] / plain, 2)
+        out[label] = r
+    if args.serve_sweep:
+        sweep = []
+        for rate in args.serve_sweep:
+            reqs = serving_requests(args, args.serve_sweep_arrivals, rate, args.seed + 1)
+            log(f"[bench] serving sweep: {len(reqs)} arrivals at {rate}/s")
+            r = serve_stream(N, reqs, args.serve_streams, args.serve_batch_max)
+            sweep.append({k: r[k] for k in ("offered_rate", "achieved_circuits_per_s", "drain_s", "latency_ms_p50",
+                                            "latency_ms_p99", "queue_ms_p50", "queue_ms_p99")}
+                         | {"class_10": r["per_class"].get("10")})
+        out["rate_sweep"] = sweep
+    return out
 
 
 def cpu_info():
@@ -474,6 +560,9 @@ def main():
 
     if args.tile_bits:
         N.set_tile_bits(*args.tile_bits)
+    if args.serve_only:
+        print(json.dumps({"serving": bench_serving(N, args)}))
+        return
     n, layers = args.qubits, args.layers
     ops = ry_rz_cz_brick(n, layers, seed=args.seed)
     n_par = sum(1 for o in ops if o[2])

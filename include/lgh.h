@@ -334,6 +334,14 @@ lgh_status lgh_server_wait(lgh_server *server, uint64_t ticket, double *expval, 
                            double *timing_ms);
 lgh_status lgh_server_stats(lgh_server *server, uint64_t *batches, uint64_t *batched_requests,
                             uint64_t *single_requests, uint64_t *budget);
+/* Batches that may run at once (default: num_streams - 1, at least 1) and
+ * circuits per batch (default 4096). */
+lgh_status lgh_server_set_limits(lgh_server *server, int32_t max_concurrent_batches, int32_t max_batch_circuits);
+/* Failure injection for tests: the next `count` pieces of work (a batch or a
+ * single circuit) fail with LGH_ERR_DEVICE before launching anything; their
+ * requests report the error from lgh_server_wait, all other requests run
+ * normally. */
+lgh_status lgh_server_inject_failures(lgh_server *server, uint64_t count);
 
 #ifdef __cplusplus
 }

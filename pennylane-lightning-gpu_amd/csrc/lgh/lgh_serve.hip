@@ -6,11 +6,16 @@
 // queue:
 //   - continuous batching: a worker that finds circuits of at most
 //     `batch_max_qubits` qubits waiting takes all of them (up to a limit) into
-//     one persistent batched launch (lgh_batch_run).  Their gate structures
+//     one persistent batched launch (lgh_batch_run).  Several batches run at
+//     once on different workers' streams (by default all workers but one), so
+//     small circuits that arrive while a batch is on the device start on the
+//     next idle worker instead of waiting for that batch.  Gate structures
 //     are compiled once into templates, cached by structure, so a stream of
 //     requests from a few circuit families costs no host compilation;
 //   - larger circuits run one per worker on device states of their own
-//     (lgh_apply, Pauli-word expectation, lgh_adjoint_jacobian).
+//     (lgh_apply, Pauli-word expectation, lgh_adjoint_jacobian), their fused
+//     passes spread over the whole device, overlapping with the batches and
+//     with each other.
 // Memory admission: every piece of work reserves the device bytes it will
 // hold (states, plus the adjoint's working copy) against a budget before it
 // starts; work that does not fit waits, and later work that fits may pass it.
@@ -100,6 +105,9 @@ struct lgh_server {
     uint64_t budget = 0;
     int batch_max_qubits = 20;
     int max_batch = 4096;
+    int max_batches = 1; // batches that may run at once
+    int max_singles = 1; // large circuits that may run at once (one worker stays free for batches)
+    int running_singles = 0;
     std::vector<void *> streams;
     std::vector<std::thread> workers;
 
@@ -108,7 +116,8 @@ struct lgh_server {
     bool stop = false;
     uint64_t next_ticket = 1;
     uint64_t reserved = 0;
-    bool batch_running = false;
+    int running_batches = 0;
+    uint64_t inject_failures = 0; // the next pieces of work fail before launching (tests)
     std::deque<Req *> small, large;
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
     // template cache by gate structure
@@ -204,11 +213,14 @@ struct lgh_server {
 
     // Under mu: the next piece of work that fits the memory budget, or
     // nothing.  Waiting small circuits go first, all of them (up to
-    // max_batch) into one batch, while no other batch runs; otherwise the
-    // oldest large circuit that fits.  Work is always admitted when nothing
-    // else holds memory.
+    // max_batch) into one batch, while fewer than max_batches batches run;
+    // otherwise the oldest large circuit that fits, while fewer than
+    // max_singles run (with several workers one always stays free for the
+    // small circuits, whose latency would otherwise be set by the largest
+    // circuits in flight).  Work is always admitted when nothing else holds
+    // memory.
     bool pick(std::vector<Req *> &batch, Req *&single, uint64_t &bytes) {
-        if (!small.empty() && !batch_running) {
+        if (!small.empty() && running_batches < max_batches) {
             uint64_t b = 0;
             size_t k = 0;
             while (k < small.size() && k < size_t(max_batch) &&
@@ -221,15 +233,17 @@ struct lgh_server {
                 batch.assign(small.begin(), small.begin() + long(k));
                 small.erase(small.begin(), small.begin() + long(k));
                 bytes = b;
-                batch_running = true;
+                ++running_batches;
                 return true;
             }
         }
+        if (running_singles >= max_singles) return false;
         for (auto it = large.begin(); it != large.end(); ++it)
             if (reserved + (*it)->bytes <= budget || reserved == 0) {
                 single = *it;
                 large.erase(it);
                 bytes = single->bytes;
+                ++running_singles;
                 return true;
             }
         return false;
@@ -248,10 +262,14 @@ struct lgh_server {
             if (single) single->t_start = now;
             for (Req *r : batch) r->t_start = now;
             reserved += bytes;
+            const bool fail = inject_failures > 0;
+            if (fail) --inject_failures;
             lk.unlock();
             lgh_status code = LGH_OK;
             std::string msg;
             try {
+                if (fail)
+                    throw Error(LGH_ERR_DEVICE, "injected failure (lgh_server_inject_failures)");
                 if (single)
                     run_single(single, stream);
                 else
@@ -268,9 +286,10 @@ struct lgh_server {
             const auto end = Clock::now();
             if (single) {
                 batch.push_back(single);
+                --running_singles;
                 ++n_single;
             } else {
-                batch_running = false;
+                --running_batches;
                 ++n_batches;
                 n_batched += batch.size();
             }
@@ -303,6 +322,8 @@ lgh_status lgh_server_create(lgh_server **out, int32_t device, uint64_t memory_b
         check_ok(lgh_device_memory(device, &free_b, &total_b));
         s->budget = memory_budget ? std::min<uint64_t>(memory_budget, free_b) : free_b / 10 * 9;
         s->batch_max_qubits = batch_max_qubits;
+        s->max_batches = std::max(1, num_streams - 1);
+        s->max_singles = std::max(1, num_streams - 1);
         for (int i = 0; i < num_streams; ++i) {
             void *st = nullptr;
             check_ok(lgh_stream_create(device, &st));
@@ -425,6 +446,26 @@ lgh_status lgh_server_wait(lgh_server *srv, uint64_t ticket, double *expval, dou
             timing_ms[1] = ms_between(r->t_start, r->t_end);
             timing_ms[2] = ms_between(r->t_submit, r->t_end);
         }
+    });
+}
+
+lgh_status lgh_server_set_limits(lgh_server *srv, int32_t max_concurrent_batches, int32_t max_batch_circuits) {
+    return guard([&] {
+        LGH_REQUIRE(srv, "null server");
+        LGH_REQUIRE(max_concurrent_batches >= 1, "max_concurrent_batches must be at least 1");
+        LGH_REQUIRE(max_batch_circuits >= 1, "max_batch_circuits must be at least 1");
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->max_batches = max_concurrent_batches;
+        srv->max_batch = max_batch_circuits;
+        srv->cv_work.notify_all();
+    });
+}
+
+lgh_status lgh_server_inject_failures(lgh_server *srv, uint64_t count) {
+    return guard([&] {
+        LGH_REQUIRE(srv, "null server");
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->inject_failures = count;
     });
 }
 

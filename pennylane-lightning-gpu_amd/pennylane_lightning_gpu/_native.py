@@ -129,6 +129,8 @@ def lib():
         L.lgh_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double), vp, C.c_int64, vp]
         L.lgh_server_stats.argtypes = [vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                        C.POINTER(C.c_uint64)]
+        L.lgh_server_set_limits.argtypes = [vp, C.c_int32, C.c_int32]
+        L.lgh_server_inject_failures.argtypes = [vp, C.c_uint64]
         _lib = L
     return _lib
 
@@ -511,6 +513,14 @@ class Server:
         tm = np.zeros(3)
         check(lib().lgh_server_wait(self.handle, int(ticket), C.byref(e), g.ctypes.data, g.size, tm.ctypes.data))
         return e.value, (g[:npar].copy() if grad else None), tuple(tm)
+
+    def set_limits(self, max_concurrent_batches, max_batch_circuits=4096):
+        """Batches that may run at once, circuits per batch."""
+        check(lib().lgh_server_set_limits(self.handle, int(max_concurrent_batches), int(max_batch_circuits)))
+
+    def inject_failures(self, count):
+        """The next ``count`` pieces of work fail before launching (tests)."""
+        check(lib().lgh_server_inject_failures(self.handle, int(count)))
 
     def stats(self):
         v = [C.c_uint64() for _ in range(4)]

@@ -90,3 +90,44 @@ def test_serve_memory_admission():
     assert res[0][0] == res[1][0] == res[2][0]
     assert srv.stats()["budget"] == budget
     srv.close()
+
+
+def test_serve_failure_injection_leaves_other_requests_intact():
+    """A piece of work that fails mid-stream (forced from the host before it
+    launches) reports its error to its own requests only; every other request
+    queued around it completes with the results it gets alone."""
+    rng = np.random.default_rng(4)
+    srv = N.Server(num_streams=2, batch_max_qubits=12)
+    large = brick(16, 2, rng)[0]
+    small = brick(10, 2, rng)[0]
+    ref_large = srv.wait(srv.submit(16, large, [("Z", 0)], True))
+    ref_small = srv.wait(srv.submit(10, small, [("Z", 0)], True))
+    srv.inject_failures(1)
+    bad = srv.submit(16, large, [("Z", 0)], True)  # the next piece of work: fails
+    with pytest.raises(N.LghError, match="injected failure"):
+        srv.wait(bad)
+    tickets = [(srv.submit(16, large, [("Z", 0)], True), ref_large) for _ in range(3)]
+    tickets += [(srv.submit(10, small, [("Z", 0)], True), ref_small) for _ in range(5)]
+    for t, ref in tickets:
+        e, g, _ = srv.wait(t)
+        assert e == ref[0]
+        assert np.array_equal(g, ref[1])
+    srv.close()
+
+
+def test_serve_concurrent_batches_match_single_batch():
+    """Small circuits served by several batches at once give the results of
+    one batch at a time, bit for bit."""
+    rng = np.random.default_rng(5)
+    reqs = [(n, brick(n, 2, rng)[0]) for n in rng.choice([9, 10, 11, 12], size=60)]
+    out = []
+    for limit in (1, 3):
+        srv = N.Server(num_streams=4, batch_max_qubits=12)
+        srv.set_limits(limit, 8)  # at most 8 circuits per batch: many batches
+        ts = [srv.submit(int(n), ops, [("Z", 0)], True) for n, ops in reqs]
+        out.append([srv.wait(t) for t in ts])
+        assert srv.stats()["batches"] >= len(reqs) // 8
+        srv.close()
+    for (e1, g1, _), (e2, g2, _) in zip(*out):
+        assert e1 == e2
+        assert np.array_equal(g1, g2)
