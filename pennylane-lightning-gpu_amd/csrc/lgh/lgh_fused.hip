@@ -26,6 +26,7 @@
 #include <atomic>
 #include <cstring>
 #include <map>
+#include <unordered_map>
 
 #include "lgh_fused_dev.hpp"
 
@@ -261,6 +262,53 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
         passes.push_back(p);
     }
     return passes;
+}
+
+// A schedule depends only on the gates' structure -- kind, targets,
+// controls, parity mask, which gates are trainable -- and on n, the register
+// bits and the fixed lane positions, never on angles or matrix entries.
+// Repeated executions of one circuit shape (optimisation loops, a serving
+// stream of one circuit family, the forward re-execution before each
+// adjoint) reuse the schedule instead of rebuilding the dependency graph.
+namespace {
+struct SchedCache {
+    std::mutex mu;
+    std::unordered_map<std::string, std::shared_ptr<const std::vector<Pass>>> map;
+};
+SchedCache &sched_cache() {
+    static SchedCache *c = new SchedCache();
+    return *c;
+}
+constexpr size_t kSchedCacheMax = 512;
+} // namespace
+
+std::shared_ptr<const std::vector<Pass>> schedule_passes_cached(const std::vector<KOp> &ops, int n,
+                                                                const std::vector<int> &trainable_slot, int rb,
+                                                                int fixed_lanes) {
+    std::string key;
+    key.reserve(16 + ops.size() * 24);
+    auto put = [&](uint64_t v) { key.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
+    put(uint64_t(n) | uint64_t(rb) << 8 | uint64_t(fixed_lanes) << 16 | uint64_t(ops.size()) << 24);
+    for (size_t i = 0; i < ops.size(); ++i) {
+        const KOp &k = ops[i];
+        const bool train = !trainable_slot.empty() && trainable_slot[i] >= 0;
+        const Use u = uses_of(k);
+        put(uint64_t(k.kind) | uint64_t(k.k) << 4 | uint64_t(train) << 12 |
+            uint64_t(k.kind == KOp::DENSE ? k.tbits[0] : 0) << 16);
+        put(u.diag);
+        put(u.nondiag);
+    }
+    SchedCache &c = sched_cache();
+    {
+        std::lock_guard<std::mutex> lk(c.mu);
+        auto it = c.map.find(key);
+        if (it != c.map.end()) return it->second;
+    }
+    auto sched = std::make_shared<const std::vector<Pass>>(schedule_passes(ops, n, trainable_slot, rb, fixed_lanes));
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (c.map.size() >= kSchedCacheMax) c.map.clear();
+    c.map.emplace(std::move(key), sched);
+    return sched;
 }
 
 } // namespace lgh
@@ -932,11 +980,11 @@ namespace lgh {
 
 void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops) {
     const int rb = g_rb_fwd.load();
-    const std::vector<Pass> passes = schedule_passes(kops, sv->n, {}, rb, fixed_lanes(sv->dtype));
+    const auto passes = schedule_passes_cached(kops, sv->n, {}, rb, fixed_lanes(sv->dtype));
     if (sv->dtype == LGH_C128)
-        run_forward<double>(sv, kops, passes, rb);
+        run_forward<double>(sv, kops, *passes, rb);
     else
-        run_forward<float>(sv, kops, passes, rb);
+        run_forward<float>(sv, kops, *passes, rb);
 }
 
 FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, lgh_dtype dtype) {
@@ -957,9 +1005,9 @@ FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, 
         }
     }
     const int rb = adjoint ? g_rb_adj.load() : g_rb_fwd.load();
-    const std::vector<Pass> passes = schedule_passes(kops, n, adjoint ? slot : std::vector<int>{}, rb, fixed_lanes(dtype));
+    const auto passes = schedule_passes_cached(kops, n, adjoint ? slot : std::vector<int>{}, rb, fixed_lanes(dtype));
     FusionPlan plan{};
-    for (const Pass &p : passes) {
+    for (const Pass &p : *passes) {
         ++plan.passes;
         if (p.standalone) {
             ++plan.standalone;
@@ -994,11 +1042,11 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
     StreamCtx *ctx = lam->ctx;
     const int n = lam->n;
     const int rb = g_rb_adj.load();
-    const std::vector<Pass> passes = schedule_passes(udag, n, slot_col, rb, fixed_lanes(lam->dtype));
+    const auto passes = schedule_passes_cached(udag, n, slot_col, rb, fixed_lanes(lam->dtype));
     const Geometry geo = geometry(n, lam->device, rb, kWavesPerCUAdjoint, g_waves_adj.load());
     // lambda still to be read from lam_src (until the first pass has run)
     const void *src = lam_src;
-    for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
+    for (auto it = passes->rbegin(); it != passes->rend(); ++it) {
         const Pass &p = *it;
         if (p.standalone) {
             if (src) {

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <functional>
+#include <memory>
 #include <vector>
 
 #include "lgh_internal.hpp"
@@ -38,6 +39,11 @@ bool fusable(const KOp &k, int n, int rb);
 // fixed).
 std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot,
                                   int rb, int fixed_lanes = 6);
+// schedule_passes through a cache keyed by the gates' structure (the
+// schedule does not depend on parameter values).
+std::shared_ptr<const std::vector<Pass>> schedule_passes_cached(const std::vector<KOp> &ops, int n,
+                                                                const std::vector<int> &trainable_slot, int rb,
+                                                                int fixed_lanes);
 // Host-side plan of the fused passes a gate sequence gets (no device use):
 // forward passes, or adjoint passes with every parametric gate trainable.
 struct FusionPlan {

@@ -321,8 +321,8 @@ constexpr int kBlock = 256;
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_gate1_pair(cx<R> *__restrict__ sv, uint64_t ngroups, Fixed fx,
                                                        uint64_t cmask, uint64_t tbit, cx<R> m00, cx<R> m01,
-                                                       cx<R> m10, cx<R> m11) {
-    const uint64_t g = xcd_block() * kBlock + threadIdx.x;
+                                                       cx<R> m10, cx<R> m11, uint64_t xcd_run) {
+    const uint64_t g = xcd_block_runs(xcd_run) * kBlock + threadIdx.x;
     if (g >= ngroups) return;
     const uint64_t i0 = expand(g, fx) | cmask;
     const cx<R> a = ldnt(sv + i0), b = ldnt(sv + (i0 | tbit));
@@ -342,6 +342,23 @@ __global__ __launch_bounds__(kBlock) void k_gate1_lane(cx<R> *__restrict__ sv, u
     const cx<R> v = i < len ? ldnt(sv + i) : cx<R>{R(0), R(0)};
     const cx<R> p = shfl_xor_cx(v, tmask);
     if (i < len) stnt(sv + i, cfma(cp, p, cmul(cs, v)));
+}
+
+// Workgroup order of the 1-target pair kernel (xcd_block_runs).  Measured on
+// MI355X with 30-qubit complex128 states (one RX per target, in place): with
+// each XCD sweeping its own eighth of the pairs, targets whose pair distance
+// is byte-address bit 25, 29 or 33 stream at 5.7-5.8 TB/s; with all XCDs in
+// one window (runs of 64 workgroups) they reach 6.3 TB/s, as do distances up
+// to bit 16 (+2 %).  Other distances are 2-4 % faster with the eighths.
+// Byte bits 24, 28 and 32 stay at 5.5-5.7 TB/s under every order tried
+// (runs of 1-2048 workgroups, index swizzles, several pairs per thread,
+// half-pairs across lanes).
+uint64_t pair_xcd_run(int n, int tbit, size_t amp_bytes) {
+    if (n < 27) return 0;
+    int byte_bit = tbit;
+    for (size_t b = amp_bytes; b > 1; b >>= 1) ++byte_bit;
+    const bool one_window = (byte_bit >= 25 && byte_bit % 4 == 1) || byte_bit <= 16;
+    return one_window ? 64 : 0;
 }
 
 template <int K> struct Offs {
@@ -445,7 +462,8 @@ template <class R> void launch_kop_t(lgh_sv *sv, const KOp &op) {
         } else {
             const uint64_t blocks = (ngroups + kBlock - 1) / kBlock;
             hipLaunchKernelGGL((k_gate1_pair<R>), dim3(unsigned(blocks)), dim3(kBlock), 0, st, data, ngroups, fx,
-                               op.cmask, 1ull << op.tbits[0], m00, m01, m10, m11);
+                               op.cmask, 1ull << op.tbits[0], m00, m01, m10, m11,
+                               pair_xcd_run(n, op.tbits[0], sizeof(cx<R>)));
         }
         LGH_LAUNCH_CHECK();
         return;

@@ -113,6 +113,18 @@ __device__ __forceinline__ uint64_t xcd_block() {
     return (blockIdx.x % 8) * (g / 8) + blockIdx.x / 8;
 }
 
+// XCD-interleaved order: the 8 XCDs take consecutive runs of `run`
+// workgroups in turn (XCD k takes runs k, k + 8, k + 16, ...), so that all
+// XCDs stream through one window of the index space at a time instead of
+// eight windows an eighth apart.  Falls back to xcd_block() unless `run`
+// divides gridDim.x / 8.
+__device__ __forceinline__ uint64_t xcd_block_runs(uint64_t run) {
+    const uint64_t g = gridDim.x;
+    if (run == 0 || g % (8 * run) != 0) return xcd_block();
+    const uint64_t k = blockIdx.x % 8, j = blockIdx.x / 8;
+    return ((j / run) * 8 + k) * run + (j % run);
+}
+
 template <class R> __device__ __forceinline__ cx<R> shfl_xor_cx(cx<R> a, int mask) {
     return cx<R>{__shfl_xor(a.re, mask, 64), __shfl_xor(a.im, mask, 64)};
 }

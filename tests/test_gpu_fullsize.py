@@ -147,3 +147,27 @@ def test_config3_tfim20_csr(dtype):
     got = c.expval(s)
     (close128 if dtype == np.complex128 else close64)(got.real, ref.real)
     assert abs(ref.imag) < 1e-9
+
+
+def test_single_gate_27q_workgroup_orders():
+    """The 1-target pair kernel on a 2 GiB state, for targets that take the
+    XCD-interleaved workgroup order (pair distance at byte-address bit 25)
+    and the per-XCD eighths (bit 26), against numpy on the same seeded
+    state; complex128, bit-for-bit up to the 1e-10 tolerance."""
+    n = 27
+    rng = np.random.default_rng(2727)
+    psi = (rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)).astype(np.complex128)
+    psi /= np.linalg.norm(psi)
+    s = N.State(n).upload(psi)
+    th = 0.731
+    c, si = math.cos(th / 2), math.sin(th / 2)
+    for b in (21, 22):
+        note(f"[fullsize] 27q RX on amplitude bit {b}")
+        s.apply([("RX", [n - 1 - b], [th], False)])
+        v = psi.reshape(1 << (n - 1 - b), 2, 1 << b)
+        a0, a1 = v[:, 0, :].copy(), v[:, 1, :].copy()
+        v[:, 0, :] = c * a0 - 1j * si * a1
+        v[:, 1, :] = -1j * si * a0 + c * a1
+        del a0, a1
+    got = s.download()
+    close128(got, psi)
