@@ -300,6 +300,29 @@ lgh_status lgh_batch_run_serial(lgh_tmpl *const *tmpls, int32_t num_circuits, lg
                                 const double *params, const int64_t *param_offset, double *expval, double *jac,
                                 const int64_t *jac_offset, void *stream);
 
+/* A batch of fixed composition (the same templates, in the same order, run
+ * again and again with new parameters: a training loop, a parameter sweep)
+ * prepared once.  Creation uploads the templates, allocates the device and
+ * pinned host buffers and captures the launch sequence (parameter upload,
+ * descriptor preparation, persistent launch, result download) in a HIP
+ * graph on a stream of the plan's own; it also autotunes the persistent
+ * launch's grid (one or two resident workgroups per CU), timing both on the
+ * plan's circuits and keeping the faster.  Each run replays the graph: the
+ * results equal lgh_batch_run's bit for bit.  No counterpart in the
+ * reference (see lgh_batch_run).  A plan must not be run from two threads at
+ * once, and its templates must outlive it. */
+typedef struct lgh_batch_plan lgh_batch_plan;
+lgh_status lgh_batch_plan_create(lgh_batch_plan **out, lgh_tmpl *const *tmpls, int32_t num_circuits,
+                                 lgh_dtype dtype, int32_t with_grad);
+/* params / param_offset / expval / jac / jac_offset as for lgh_batch_run;
+ * jac must be NULL for a plan created without gradients. */
+lgh_status lgh_batch_plan_run(lgh_batch_plan *plan, const double *params, const int64_t *param_offset,
+                              double *expval, double *jac, const int64_t *jac_offset);
+/* The tuned grid (workgroups per CU) and the timed candidates (ms, one and
+ * two workgroups per CU). */
+lgh_status lgh_batch_plan_info(const lgh_batch_plan *plan, int32_t *grid_per_cu, double *tuned_ms);
+lgh_status lgh_batch_plan_destroy(lgh_batch_plan *plan);
+
 /* ---- serving ------------------------------------------------------------
  * No counterpart in the reference, where each PennyLane execution calls the
  * device synchronously (lightning_gpu.py:519-555, :638-752); one request is

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -123,6 +124,10 @@ def lib():
         L.lgh_tmpl_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.lgh_batch_run.argtypes = [vp, C.c_int32, C.c_int, vp, vp, vp, vp, vp, vp]
         L.lgh_batch_run_serial.argtypes = [vp, C.c_int32, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.lgh_batch_plan_create.argtypes = [C.POINTER(vp), vp, C.c_int32, C.c_int, C.c_int32]
+        L.lgh_batch_plan_run.argtypes = [vp, vp, vp, vp, vp, vp]
+        L.lgh_batch_plan_info.argtypes = [vp, C.POINTER(C.c_int32), vp]
+        L.lgh_batch_plan_destroy.argtypes = [vp]
         L.lgh_server_create.argtypes = [C.POINTER(vp), C.c_int32, C.c_uint64, C.c_int32, C.c_int32]
         L.lgh_server_destroy.argtypes = [vp]
         L.lgh_server_submit.argtypes = [vp, C.POINTER(Request), C.POINTER(C.c_uint64)]
@@ -173,39 +178,65 @@ def device_count() -> int:
     return c.value
 
 
+# numpy view of struct lgh_op (same field order, offsets and size)
+OP_DTYPE = np.dtype([("gate", np.int32), ("num_wires", np.int32), ("num_ctrl", np.int32), ("inverse", np.int32),
+                     ("wires", np.uint64), ("params", np.float64, (3,)), ("matrix", np.uint64),
+                     ("num_params", np.int32), ("reserved", np.int32)])
+assert OP_DTYPE.itemsize == C.sizeof(Op)
+_STATEPREP_NAMES = ("StatePrep", "QubitStateVector", "BasisState")
+
+
 def make_ops(ops):
     """Pack ``(name, wires, params, inverse[, matrix])`` tuples into an lgh_op array.
 
-    Returns the array and a list of buffers that must outlive the call.
+    Returns the array (a ctypes view of a numpy record array) and a list of
+    buffers that must outlive the call.  Wires of all ops share one int32
+    array; the records are filled column by column.
     """
-    arr = (Op * max(1, len(ops)))()
-    keep = []
+    n = len(ops)
+    rec = np.zeros(max(1, n), dtype=OP_DTYPE)
+    keep = [rec]
+    if n == 0:
+        return (Op * 1).from_buffer(rec), keep
+    gate = np.empty(n, dtype=np.int32)
+    nwires = np.empty(n, dtype=np.int64)
+    params = np.zeros((n, 3))
+    npar = np.empty(n, dtype=np.int32)
+    inv = np.empty(n, dtype=np.int32)
+    flat = []
+    mat_id = GATE_ID["Matrix"]
     for i, op in enumerate(ops):
-        name, wires, params, inverse = op[:4]
-        mat = op[4] if len(op) > 4 else None
-        o = arr[i]
-        if name in ("StatePrep", "QubitStateVector", "BasisState"):
-            o.gate = GATE_STATEPREP
-        elif name in GATE_ID and name != "Matrix":
-            o.gate = GATE_ID[name]
-        elif mat is not None:
-            o.gate = GATE_ID["Matrix"]
-        else:
-            raise ValueError(f"Currently unsupported gate: {name}")
-        w = (C.c_int32 * max(1, len(wires)))(*[int(x) for x in wires])
-        keep.append(w)
-        o.wires = C.cast(w, C.POINTER(C.c_int32))
-        o.num_wires = len(wires)
-        o.inverse = int(bool(inverse))
-        for k, p in enumerate(list(params)[:3]):
-            o.params[k] = float(p)
-        o.num_params = len(params)
-        if mat is not None and o.gate == GATE_ID["Matrix"]:
-            m = np.ascontiguousarray(np.asarray(mat, dtype=np.complex128).ravel()).view(np.float64)
-            keep.append(m)
-            o.matrix = m.ctypes.data_as(C.POINTER(C.c_double))
-            o.num_ctrl = 0
-    return arr, keep
+        name, wires, ps, inverse = op[:4]
+        g = GATE_ID.get(name)
+        if g is None or g == mat_id:
+            if name in _STATEPREP_NAMES:
+                g = GATE_STATEPREP
+            elif len(op) > 4 and op[4] is not None:
+                g = mat_id
+                m = np.ascontiguousarray(np.asarray(op[4], dtype=np.complex128).ravel()).view(np.float64)
+                keep.append(m)
+                rec["matrix"][i] = m.ctypes.data
+            else:
+                raise ValueError(f"Currently unsupported gate: {name}")
+        gate[i] = g
+        nwires[i] = len(wires)
+        flat.extend(wires)
+        k = len(ps)
+        npar[i] = k
+        if k:
+            params[i, :min(k, 3)] = list(ps)[:3]
+        inv[i] = 1 if inverse else 0
+    wires_arr = np.asarray(flat if flat else [0], dtype=np.int32)
+    keep.append(wires_arr)
+    starts = np.zeros(n, dtype=np.int64)
+    starts[1:] = np.cumsum(nwires[:-1])
+    rec["gate"][:n] = gate
+    rec["num_wires"][:n] = nwires
+    rec["inverse"][:n] = inv
+    rec["wires"][:n] = np.uint64(wires_arr.ctypes.data) + starts.astype(np.uint64) * np.uint64(4)
+    rec["params"][:n] = params
+    rec["num_params"][:n] = npar
+    return (Op * n).from_buffer(rec), keep
 
 
 def gate_matrix(name, wires, params=(), inverse=False, matrix=None):
@@ -439,6 +470,7 @@ class Template:
         npar, nq = C.c_int32(), C.c_int32()
         check(lib().lgh_tmpl_info(self.handle, C.byref(npar), C.byref(nq)))
         self.num_params, self.num_qubits = npar.value, nq.value
+        self._plans = weakref.WeakSet()  # batch plans that use this template
 
     def __del__(self):
         try:
@@ -447,6 +479,10 @@ class Template:
             pass
 
     def close(self):
+        """Destroys the template; batch plans built on it are closed first
+        (a plan must not outlive its templates)."""
+        for plan in list(getattr(self, "_plans", ())):
+            plan.close()
         if getattr(self, "handle", None) and self.handle.value:
             lib().lgh_tmpl_destroy(self.handle)
             self.handle = C.c_void_p()
@@ -476,6 +512,63 @@ def batch_run(templates, params, grad=True, dtype=np.complex128, serial=False, s
              jac.ctypes.data if grad else None, offs.ctypes.data if grad else None, stream))
     jacs = [jac[o:o + p.size].copy() for o, p in zip(offs[:nc], flat)] if grad else None
     return ev[:nc].copy(), jacs
+
+
+class BatchPlan:
+    """A batch of fixed composition prepared once (lgh_batch_plan_*): the
+    launch sequence is a captured HIP graph replayed by every ``run``, and
+    the persistent launch's grid is autotuned at creation."""
+
+    def __init__(self, templates, grad=True, dtype=np.complex128):
+        self.templates = list(templates)
+        self.nc = len(self.templates)
+        self.grad = bool(grad)
+        self.handle = C.c_void_p()
+        self._hs = (C.c_void_p * max(1, self.nc))(*[t.handle.value for t in self.templates])
+        code = LGH_C128 if np.dtype(dtype) == np.complex128 else LGH_C64
+        check(lib().lgh_batch_plan_create(C.byref(self.handle), self._hs, self.nc, code, int(self.grad)))
+        for t in set(self.templates):
+            t._plans.add(self)
+        sizes = [t.num_params for t in self.templates]
+        self.offs = np.zeros(max(1, self.nc), dtype=np.int64)
+        if self.nc:
+            self.offs[1:self.nc] = np.cumsum(sizes)[:-1]
+        self.total = int(sum(sizes))
+
+    def run(self, params):
+        """params: (num_circuits, P) array or a list of 1-D arrays; returns
+        (expvals, jacobians or None) like ``batch_run``."""
+        flat = np.ascontiguousarray(np.concatenate([np.asarray(p, dtype=np.float64).ravel() for p in params])
+                                    if self.total else np.zeros(1))
+        if flat.size != max(1, self.total) and self.total:
+            raise ValueError(f"plan takes {self.total} parameters in total, got {flat.size}")
+        if not self.handle.value:
+            raise LghError(1, "batch plan is closed")
+        ev = np.zeros(max(1, self.nc))
+        jac = np.zeros(max(1, self.total)) if self.grad else None
+        check(lib().lgh_batch_plan_run(self.handle, flat.ctypes.data, self.offs.ctypes.data, ev.ctypes.data,
+                                       jac.ctypes.data if self.grad else None,
+                                       self.offs.ctypes.data if self.grad else None))
+        sizes = [t.num_params for t in self.templates]
+        jacs = [jac[o:o + k].copy() for o, k in zip(self.offs[:self.nc], sizes)] if self.grad else None
+        return ev[:self.nc].copy(), jacs
+
+    def info(self):
+        g = C.c_int32()
+        ms = np.zeros(2)
+        check(lib().lgh_batch_plan_info(self.handle, C.byref(g), ms.ctypes.data))
+        return {"grid_per_cu": g.value, "tuned_ms": ms.tolist()}
+
+    def close(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            check(lib().lgh_batch_plan_destroy(self.handle))
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- serving

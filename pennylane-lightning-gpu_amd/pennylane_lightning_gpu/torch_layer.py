@@ -7,7 +7,8 @@ incoming gradient (pennylane_lightning_gpu/lightning_gpu.py:638-752 and
 
 - ``BatchedExpval``: a batch of circuits over one template (a gate structure
   with parameter slots and a Pauli-word observable, up to 20 qubits).  The
-  forward is one persistent batched launch (lgh_batch_run) that returns
+  forward is one persistent batched launch (a replay of a cached batch plan,
+  lgh_batch_plan_run, per template and batch size) that returns
   <P> for every circuit and, when a gradient is needed, the adjoint
   Jacobian of every circuit in the same launch; the backward is
   grad_out[:, None] * J.
@@ -19,6 +20,8 @@ Parameters may live on any torch device; the values cross to the host
 """
 from __future__ import annotations
 
+from collections import OrderedDict
+
 import numpy as np
 import torch
 
@@ -27,6 +30,25 @@ from . import _native as N
 
 def _host64(t: torch.Tensor) -> np.ndarray:
     return t.detach().to("cpu", torch.float64).contiguous().numpy()
+
+
+# Batch plans (captured launch sequence, tuned grid) of recent batch shapes:
+# a training loop calls the same template with the same batch size every
+# step, so each step replays a graph instead of rebuilding the launch.
+_PLANS: "OrderedDict" = OrderedDict()
+_PLANS_MAX = 8
+
+
+def _plan(template: "N.Template", batch: int, grad: bool, dtype) -> "N.BatchPlan":
+    key = (template.handle.value, batch, bool(grad), np.dtype(dtype).name)
+    plan = _PLANS.get(key)
+    if plan is None or not plan.handle.value:
+        plan = N.BatchPlan([template] * batch, grad=grad, dtype=dtype)
+        _PLANS[key] = plan
+        while len(_PLANS) > _PLANS_MAX:
+            _PLANS.popitem(last=False)
+    _PLANS.move_to_end(key)
+    return plan
 
 
 class BatchedExpval(torch.autograd.Function):
@@ -38,7 +60,8 @@ class BatchedExpval(torch.autograd.Function):
             raise ValueError(f"params must be (batch, {template.num_params})")
         p = _host64(params)
         need = params.requires_grad
-        ev, jac = N.batch_run([template] * p.shape[0], list(p), grad=need, dtype=dtype)
+        ev, jac = _plan(template, p.shape[0], need, dtype).run(list(p)) if p.shape[0] else \
+            N.batch_run([], [], grad=need, dtype=dtype)
         if need:
             J = torch.from_numpy(np.stack(jac) if jac else np.zeros((0, template.num_params)))
             ctx.save_for_backward(J.to(params.device, params.dtype))
