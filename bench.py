@@ -33,6 +33,19 @@ for _p in (ROOT, PKG_DIR):
 HBM_PEAK_GBS = 8000.0
 
 
+def kernel_sources_sha16():
+    """Hash of the sources of the fused-pass kernels (device code and the
+    headers it includes): a PMC traffic figure measured on other sources is
+    stale."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for rel in ("lgh_fused.hip", "lgh_fused_dev.hpp", "lgh_internal.hpp"):
+        with open(os.path.join(PKG_DIR, "csrc", "lgh", rel), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -51,6 +64,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=1337)
     ap.add_argument("--tile-bits", type=int, nargs=2, default=None, metavar=("FWD", "ADJ"),
                     help="register bits of the fused forward / adjoint passes (library default 4 3)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default tile heights and grids instead of autotuning them")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-measure", action="store_true")
@@ -113,12 +128,17 @@ class StreamTimer:
         return start.elapsed_time(end) / reps  # ms
 
 
-def bench_batched(N, args):
+def bench_batched(N, args, dtype):
     """Batched workload: circuits with n ~ U{8..20} qubits (seeded), each 3 x
     [RX, RY, RZ on every wire; CNOT ring] (12n gates, 9n parameters), <Z_0>
-    and its gradient; complex128.  One step = all circuits in one persistent
-    launch.  Compared with the per-circuit launch form on a sample of the
-    same circuits."""
+    and its gradient.  One step = all circuits in one persistent launch,
+    replayed from a batch plan (captured HIP graph, grid autotuned at plan
+    creation).  Compared with the per-circuit launch form on a sample of the
+    same circuits.  Roofline of the batched kernel: algorithmic bytes
+    (2 F + 4 R + 4) S_c per circuit (F forward and R reverse passes over
+    states of S_c bytes, the observable pass and the initial state) over the
+    kernel time; states of up to 2^20 amplitudes may stay in L2 / MALL
+    between passes, so HBM moves fewer bytes than this."""
     import numpy as np
 
     from pennylane_lightning_gpu.workloads import batch_widths, ring_template
@@ -131,20 +151,29 @@ def bench_batched(N, args):
         tm[w] = N.Template(w, ops, pidx, [("Z", 0)])
     ts = [tm[w] for w in widths]
     thetas = [rng.uniform(0, 2 * np.pi, t.num_params) for t in ts]
-    log(f"[bench] batched: {len(ts)} circuits, n in [{min(widths)}, {max(widths)}], forward + gradient")
-    N.batch_run(ts, thetas)  # warm-up
+    elem = 16 if np.dtype(dtype) == np.complex128 else 8
+    alg_bytes = 0
+    for t in ts:
+        f, r, q = t.passes()
+        alg_bytes += (2 * f + 4 * r + 4) * elem * (1 << q)
+    name = np.dtype(dtype).name
+    log(f"[bench] batched {name}: {len(ts)} circuits, n in [{min(widths)}, {max(widths)}], forward + gradient")
+    t0 = time.perf_counter()
+    plan = N.BatchPlan(ts, grad=True, dtype=dtype)
+    t_plan = time.perf_counter() - t0
+    plan.run(thetas)  # warm-up
     N.profile_enable(True)
     steps = []
     for i in range(args.batch_steps):
         t0 = time.perf_counter()
-        ev, jac = N.batch_run(ts, thetas)
+        ev, jac = plan.run(thetas)
         steps.append(time.perf_counter() - t0)
-        log(f"[bench] batched step {i}: {steps[-1] * 1e3:.1f} ms")
+        log(f"[bench] batched {name} step {i}: {steps[-1] * 1e3:.1f} ms")
     kn, kms = N.profile_read("batch")
     N.profile_enable(False)
     k = min(args.batch_serial_sample, len(ts))
     t0 = time.perf_counter()
-    ev_s, jac_s = N.batch_run(ts[:k], thetas[:k], serial=True)
+    ev_s, jac_s = N.batch_run(ts[:k], thetas[:k], serial=True, dtype=dtype)
     t_serial = time.perf_counter() - t0
     diff = max(float(np.max(np.abs(ev[:k] - ev_s))),
                max(float(np.max(np.abs(a - b))) for a, b in zip(jac[:k], jac_s)))
@@ -153,17 +182,27 @@ def bench_batched(N, args):
     # the sample's share of the full batch's work, by circuit cost ~ n 2^n
     cost = np.array([w * 2.0 ** w for w in widths])
     serial_full = t_serial * cost.sum() / cost[:k].sum()
-    return {
-        "circuits": len(ts), "qubits": [min(widths), max(widths)], "dtype": "c128",
+    kernel_ms = kms / max(1, kn)
+    out = {
+        "circuits": len(ts), "qubits": [min(widths), max(widths)], "dtype": name,
         "circuits_per_s": round(cps, 1),
         "step_ms_p50": round(float(np.percentile(st, 50)) * 1e3, 2),
         "step_ms_p99": round(float(np.percentile(st, 99)) * 1e3, 2),
-        "kernel_ms_per_step": round(kms / max(1, kn), 2),
+        "kernel_ms_per_step": round(kernel_ms, 2),
+        "plan": plan.info() | {"create_s": round(t_plan, 3)},
+        "roofline": {"bound": "hbm", "kernel": f"k_batch<{'double' if elem == 16 else 'float'}>",
+                     "algorithmic_bytes": int(alg_bytes),
+                     "achieved": round(alg_bytes / (kernel_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(alg_bytes / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3)},
         "per_circuit_form": {"sample": k, "seconds": round(t_serial, 3),
                              "circuits_per_s_est": round(len(ts) / serial_full, 1),
                              "max_abs_diff_vs_batched": diff},
         "speedup_vs_per_circuit": round(serial_full / float(np.median(st)), 1),
     }
+    plan.close()
+    for t in tm.values():
+        t.close()
+    return out
 
 
 def bench_measurements(N, args):
@@ -568,6 +607,14 @@ def main():
     n_par = sum(1 for o in ops if o[2])
     state_bytes = 16 * (1 << n)
 
+    # one-time setup: the autotuner picks tile height and grid of the fused
+    # passes for this state size on the headline circuit (outside the timed steps)
+    tune = None
+    if not args.no_autotune and not args.tile_bits:
+        log(f"[bench] autotuning the fused passes on the {n}-qubit headline circuit")
+        tune = N.autotune_fused(n, ops)
+        log(f"[bench] tuned: forward {tune['forward']}, adjoint {tune['adjoint']}")
+
     # ---------------------------------------------------------------- headline: adjoint Jacobian
     sv = ops_mod.LightningGPU_C128(n)
     adj = ops_mod.AdjointJacobianGPU_C128()
@@ -612,11 +659,17 @@ def main():
     fused_bytes = 4 * state_bytes
     fused_gbs = fused_bytes / (adj_avg * 1e-3) / 1e9
     fwd_avg = fwd_ms / max(1, fwd_n)
-    traffic = None
+    # HBM bytes per launch from the committed PMC summary, used only if it
+    # was measured on the current kernel sources
+    traffic, traffic_note = None, None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile) and n == 28:
         with open(tfile) as f:
-            traffic = json.load(f).get(f"k_fused_adj<double>@{n}q", {}).get("traffic_bytes")
+            entry = json.load(f).get(f"k_fused_adj<double>@{n}q", {})
+        if entry.get("kernel_sources_sha16") == kernel_sources_sha16():
+            traffic = entry.get("traffic_bytes")
+        else:
+            traffic_note = "PMC summary predates the current kernel sources: not used"
     kernel_stats = {
         "k_fused_adj": {"launches_per_step": adj_n / args.steps, "avg_ms": round(adj_avg, 4),
                         "ms_per_step": round(adj_ms / args.steps, 2), "bytes_per_launch": fused_bytes,
@@ -629,7 +682,8 @@ def main():
     # ---------------------------------------------------------------- batched circuits
     batched = None
     if not args.no_batch:
-        batched = bench_batched(N, args)
+        batched = bench_batched(N, args, np.complex128)
+        batched["complex64"] = bench_batched(N, args, np.complex64)
     serving = None
     if not args.no_serve:
         serving = bench_serving(N, args)
@@ -688,9 +742,10 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "k_fused_adj<double> (fused adjoint pass: lambda + 1 observable state)",
                      "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(fused_gbs / HBM_PEAK_GBS, 3), "traffic": traffic,
+                     "frac": round(fused_gbs / HBM_PEAK_GBS, 3), "traffic": traffic, "traffic_note": traffic_note,
                      "bytes_per_launch": fused_bytes, "ms_per_launch": round(adj_avg, 4)},
         "cpu_baseline": cpu,
+        "autotune": tune,
         "kernels": kernel_stats,
         "batched": batched,
         "serving": serving,

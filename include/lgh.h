@@ -137,6 +137,30 @@ lgh_status lgh_set_tile_bits(int forward, int adjoint);
  * amplitudes, so a small cap exercises the multi-tile path at any size.  No
  * counterpart in the reference; a tuning knob used by tests. */
 lgh_status lgh_set_pass_waves(int64_t forward, int64_t adjoint);
+/* Autotuner of the fused passes (no counterpart in the reference): times
+ * `ops` (or, with num_ops = 0, two layers of RY and RZ on every wire plus a
+ * CZ brick) on fresh states of num_qubits (10..34) qubits under every
+ * candidate tile height (register bits 3 and 4) and grid (forward: 512 or
+ * 4096 wavefronts per CU; adjoint: 128, 256 or 512), the forward passes
+ * (lgh_apply) and the adjoint sweep of <Z_0> (lgh_adjoint_jacobian, every
+ * parametrised gate trainable) separately, and keeps the fastest of each for
+ * (device, num_qubits, dtype): later lgh_apply / lgh_adjoint_jacobian calls
+ * on such states use it.  Other tile heights group the gates into other
+ * passes: results change within rounding.  lgh_set_tile_bits,
+ * lgh_set_pass_waves and lgh_autotune_clear drop every tuned choice. */
+typedef struct lgh_tune_result {
+    int32_t forward_tile_bits, forward_waves_per_cu, adjoint_tile_bits, adjoint_waves_per_cu;
+    double forward_ms, adjoint_ms;                 /* the chosen candidates */
+    double default_forward_ms, default_adjoint_ms; /* the defaults in force before */
+    int32_t candidates;                            /* timed candidates, in order */
+    int32_t cand_dir[16];                          /* 0: forward, 1: adjoint */
+    int32_t cand_tile_bits[16];
+    int32_t cand_waves_per_cu[16];
+    double cand_ms[16];
+} lgh_tune_result;
+lgh_status lgh_autotune_fused(int device, int32_t num_qubits, lgh_dtype dtype, const lgh_op *ops, int64_t num_ops,
+                              lgh_tune_result *out);
+lgh_status lgh_autotune_clear(void);
 /* Plan of the fused passes that `ops` get on `num_qubits` qubits, computed on
  * the host without touching a device: passes in total, of those the
  * standalone ones (gates that fit no fused pass), and over the fused passes
@@ -283,6 +307,12 @@ lgh_status lgh_tmpl_create(lgh_tmpl **out, int32_t num_qubits, const lgh_op *ops
 lgh_status lgh_tmpl_destroy(lgh_tmpl *tmpl);
 /* number of parameters (1 + the largest param_index) and of qubits */
 lgh_status lgh_tmpl_info(const lgh_tmpl *tmpl, int32_t *num_params, int32_t *num_qubits);
+/* Passes a batched run makes over the circuit's states: forward passes,
+ * reverse (adjoint) passes, and the qubits of the states it runs on (at
+ * least 9; smaller circuits run with idle low-order wires).  For byte
+ * accounting of the batched kernel. */
+lgh_status lgh_tmpl_passes(const lgh_tmpl *tmpl, int32_t *forward_passes, int32_t *reverse_passes,
+                           int32_t *state_qubits);
 
 /* Circuit c (0 <= c < num_circuits) is tmpls[c] with parameters
  * params[param_offset[c] ...] (host memory, float64).  expval[c] = <P>; if

@@ -1034,6 +1034,16 @@ lgh_status lgh_tmpl_info(const lgh_tmpl *tmpl, int32_t *num_params, int32_t *num
     });
 }
 
+lgh_status lgh_tmpl_passes(const lgh_tmpl *tmpl, int32_t *forward_passes, int32_t *reverse_passes,
+                           int32_t *state_qubits) {
+    return guard([&] {
+        LGH_REQUIRE(tmpl, "null template");
+        if (forward_passes) *forward_passes = int32_t(tmpl->fwd.size());
+        if (reverse_passes) *reverse_passes = int32_t(tmpl->rev.size());
+        if (state_qubits) *state_qubits = tmpl->nq;
+    });
+}
+
 lgh_status lgh_batch_run(lgh_tmpl *const *tmpls, int32_t num_circuits, lgh_dtype dtype, const double *params,
                          const int64_t *param_offset, double *expval, double *jac, const int64_t *jac_offset,
                          void *stream) {

@@ -25,7 +25,10 @@
 #include <complex>
 #include <atomic>
 #include <cstring>
+#include <array>
+#include <chrono>
 #include <map>
+#include <tuple>
 #include <unordered_map>
 
 #include "lgh_fused_dev.hpp"
@@ -48,6 +51,46 @@ int fixed_lanes(lgh_dtype dt) { return dt == LGH_C128 ? 3 : 4; }
 std::atomic<int> g_rb_fwd{kRBForward}, g_rb_adj{kRBAdjoint};
 // caps on the wavefronts of a pass's grid (0: the per-CU defaults below)
 std::atomic<int64_t> g_waves_fwd{0}, g_waves_adj{0};
+
+// Wavefronts per CU of a pass's grid: many more than fit at once (12 per CU
+// for the default kernels), each taking a short run of consecutive tiles, so
+// that the dispatcher keeps every CU fed to the end of the pass.  Measured at
+// 28 qubits: 192-256 per CU (8-11 tiles each) beat 32 (64 tiles each) by
+// 6-7 % for the adjoint pass, whose per-workgroup overlap reduction makes
+// still shorter runs cost more; the forward pass is fastest with one tile
+// per wavefront (5 % over 32 tiles).  The autotuner may pick other values
+// per state size (at most kMaxWavesPerCUAdjoint for the adjoint).
+constexpr int kWavesPerCUForward = 4096; // one tile per wavefront up to 30 qubits
+constexpr int kWavesPerCUAdjoint = 256;
+constexpr int kMaxWavesPerCUAdjoint = 512;
+
+// Tile height and grid of the fused passes per (device, qubits, dtype,
+// direction): chosen by lgh_autotune_fused, else the defaults (tile bits of
+// lgh_set_tile_bits, the wavefronts per CU above).
+struct FusedCfg {
+    int rb;
+    int waves_per_cu;
+};
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int>, FusedCfg> g_tuned;
+// the autotuner's candidate, for the thread that times it
+thread_local const FusedCfg *t_force[2] = {nullptr, nullptr};
+
+void clear_tuned() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned.clear();
+}
+
+// dir 0: forward passes, 1: adjoint passes
+FusedCfg fused_cfg(int device, int n, lgh_dtype dt, int dir) {
+    if (t_force[dir]) return *t_force[dir];
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tuned.find(std::make_tuple(device, n, int(dt), dir));
+        if (it != g_tuned.end()) return it->second;
+    }
+    return dir == 0 ? FusedCfg{g_rb_fwd.load(), kWavesPerCUForward} : FusedCfg{g_rb_adj.load(), kWavesPerCUAdjoint};
+}
 
 struct Use {
     uint64_t diag = 0;    // bits used diagonally (controls, phases)
@@ -72,12 +115,14 @@ void set_tile_bits(int forward, int adjoint) {
     LGH_REQUIRE((forward == 3 || forward == 4) && (adjoint == 3 || adjoint == 4), "tile bits are 3 or 4");
     g_rb_fwd.store(forward);
     g_rb_adj.store(adjoint);
+    clear_tuned();
 }
 int fusion_enabled() { return g_fusion.load(); }
 void set_pass_waves(int64_t forward, int64_t adjoint) {
     LGH_REQUIRE(forward >= 0 && adjoint >= 0, "wavefront caps are >= 0 (0: default)");
     g_waves_fwd.store(forward);
     g_waves_adj.store(adjoint);
+    clear_tuned();
 }
 
 bool fusable(const KOp &k, int n, int rb) {
@@ -926,16 +971,6 @@ struct Geometry {
     uint64_t ntiles, chunk;
     unsigned blocks;
 };
-// Wavefronts per CU of a pass's grid: many more than fit at once (12 per CU
-// for the default kernels), each taking a short run of consecutive tiles, so
-// that the dispatcher keeps every CU fed to the end of the pass.  Measured at
-// 28 qubits: 192-256 per CU (8-11 tiles each) beat 32 (64 tiles each) by
-// 6-7 % for the adjoint pass, whose per-workgroup overlap reduction makes
-// still shorter runs cost more; the forward pass is fastest with one tile
-// per wavefront (5 % over 32 tiles).
-constexpr int kWavesPerCUForward = 4096; // one tile per wavefront up to 30 qubits
-constexpr int kWavesPerCUAdjoint = 256;
-
 // cap: at most that many wavefronts (0: no cap beyond waves_per_cu)
 Geometry geometry(int n, int device, int rb, int waves_per_cu, int64_t cap = 0) {
     Geometry g{};
@@ -951,9 +986,10 @@ Geometry geometry(int n, int device, int rb, int waves_per_cu, int64_t cap = 0) 
 }
 
 template <class R>
-void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pass> &passes, int rb) {
+void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pass> &passes, FusedCfg cfg) {
     StreamCtx *ctx = sv->ctx;
-    const Geometry geo = geometry(sv->n, sv->device, rb, kWavesPerCUForward, g_waves_fwd.load());
+    const int rb = cfg.rb;
+    const Geometry geo = geometry(sv->n, sv->device, rb, cfg.waves_per_cu, g_waves_fwd.load());
     for (const Pass &p : passes) {
         if (p.standalone) {
             launch_kop(sv, kops[size_t(p.ops[0])]);
@@ -979,12 +1015,12 @@ void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pas
 namespace lgh {
 
 void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops) {
-    const int rb = g_rb_fwd.load();
-    const auto passes = schedule_passes_cached(kops, sv->n, {}, rb, fixed_lanes(sv->dtype));
+    const FusedCfg cfg = fused_cfg(sv->device, sv->n, sv->dtype, 0);
+    const auto passes = schedule_passes_cached(kops, sv->n, {}, cfg.rb, fixed_lanes(sv->dtype));
     if (sv->dtype == LGH_C128)
-        run_forward<double>(sv, kops, *passes, rb);
+        run_forward<double>(sv, kops, *passes, cfg);
     else
-        run_forward<float>(sv, kops, *passes, rb);
+        run_forward<float>(sv, kops, *passes, cfg);
 }
 
 FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, lgh_dtype dtype) {
@@ -1004,7 +1040,7 @@ FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, 
             slot.push_back(-1);
         }
     }
-    const int rb = adjoint ? g_rb_adj.load() : g_rb_fwd.load();
+    const int rb = fused_cfg(0, n, dtype, adjoint ? 1 : 0).rb;
     const auto passes = schedule_passes_cached(kops, n, adjoint ? slot : std::vector<int>{}, rb, fixed_lanes(dtype));
     FusionPlan plan{};
     for (const Pass &p : *passes) {
@@ -1041,9 +1077,10 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
     // are scheduled on that order and executed backwards.
     StreamCtx *ctx = lam->ctx;
     const int n = lam->n;
-    const int rb = g_rb_adj.load();
+    const FusedCfg cfg = fused_cfg(lam->device, n, lam->dtype, 1);
+    const int rb = cfg.rb;
     const auto passes = schedule_passes_cached(udag, n, slot_col, rb, fixed_lanes(lam->dtype));
-    const Geometry geo = geometry(n, lam->device, rb, kWavesPerCUAdjoint, g_waves_adj.load());
+    const Geometry geo = geometry(n, lam->device, rb, cfg.waves_per_cu, g_waves_adj.load());
     // lambda still to be read from lam_src (until the first pass has run)
     const void *src = lam_src;
     for (auto it = passes->rbegin(); it != passes->rend(); ++it) {
@@ -1110,7 +1147,7 @@ void fused_adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std:
 size_t fused_partials_bytes(int n, int device) {
     // sized for the smaller tile, which has more workgroups
     if (n < kLaneBits + 3) return 0;
-    return sizeof(double) * size_t(kMaxSlots) * geometry(n, device, 3, kWavesPerCUAdjoint).blocks;
+    return sizeof(double) * size_t(kMaxSlots) * geometry(n, device, 3, kMaxWavesPerCUAdjoint).blocks;
 }
 
 } // namespace lgh
@@ -1147,4 +1184,180 @@ extern "C" lgh_status lgh_fusion_plan(int32_t num_qubits, const lgh_op *ops, int
         *runs = plan.runs;
         *run_members = plan.run_members;
     });
+}
+
+// ---------------------------------------------------------------- autotuner
+//
+// Times the fused passes of one circuit under every candidate tile height
+// (register bits 3 and 4) and grid (wavefronts per CU) on states of the
+// requested size, forward and adjoint separately, and keeps the fastest per
+// (device, qubits, dtype, direction).  Results do not depend on the grid;
+// other tile heights schedule the gates into other passes, which changes the
+// rounding of the results but not their values beyond it.
+
+namespace {
+
+struct ForceCfg {
+    int dir;
+    ForceCfg(int d, const FusedCfg *c) : dir(d) { t_force[d] = c; }
+    ~ForceCfg() { t_force[dir] = nullptr; }
+};
+
+void ok(lgh_status s) {
+    if (s != LGH_OK) throw Error(s, lgh_last_error());
+}
+
+struct Owned {
+    lgh_sv *sv = nullptr;
+    ~Owned() {
+        if (sv) lgh_sv_destroy(sv);
+    }
+};
+
+// milliseconds of fn() on `stream`, best of `reps` after one warm-up
+template <class F> double time_on(hipStream_t stream, int reps, F &&fn) {
+    hipEvent_t a = nullptr, b = nullptr;
+    LGH_HIP(hipEventCreate(&a));
+    struct Ev {
+        hipEvent_t &a, &b;
+        ~Ev() {
+            if (a) (void)hipEventDestroy(a);
+            if (b) (void)hipEventDestroy(b);
+        }
+    } ev{a, b};
+    LGH_HIP(hipEventCreate(&b));
+    fn();
+    float best = 1e30f;
+    for (int r = 0; r < reps; ++r) {
+        LGH_HIP(hipEventRecord(a, stream));
+        fn();
+        LGH_HIP(hipEventRecord(b, stream));
+        LGH_HIP(hipEventSynchronize(b));
+        float ms = 0.0f;
+        LGH_HIP(hipEventElapsedTime(&ms, a, b));
+        best = std::min(best, ms);
+    }
+    return double(best);
+}
+
+} // namespace
+
+extern "C" lgh_status lgh_autotune_fused(int device, int32_t num_qubits, lgh_dtype dtype, const lgh_op *ops,
+                                         int64_t num_ops, lgh_tune_result *out) {
+    return guard([&] {
+        LGH_REQUIRE(out, "null pointer");
+        LGH_REQUIRE(dtype == LGH_C64 || dtype == LGH_C128, "unknown dtype");
+        LGH_REQUIRE(num_qubits >= kLaneBits + 4 && num_qubits <= 34, "autotuning needs 10 to 34 qubits");
+        LGH_REQUIRE(num_ops >= 0 && (num_ops == 0 || ops), "bad op list");
+        *out = lgh_tune_result{};
+        const int n = num_qubits;
+        // the circuit: as given, or 2 layers of RY, RZ on every wire and a CZ brick
+        std::vector<lgh_op> circ;
+        std::vector<std::array<int32_t, 2>> wire_store;
+        if (num_ops > 0) {
+            for (int64_t i = 0; i < num_ops; ++i)
+                if (ops[i].gate >= 0) circ.push_back(ops[i]);
+        } else {
+            wire_store.reserve(size_t(4 * n));
+            for (int layer = 0; layer < 2; ++layer) {
+                for (int w = 0; w < n; ++w)
+                    for (int g : {int(LGH_GATE_RY), int(LGH_GATE_RZ)}) {
+                        wire_store.push_back({w, 0});
+                        lgh_op o{};
+                        o.gate = g;
+                        o.num_wires = 1;
+                        o.wires = wire_store.back().data();
+                        o.params[0] = 0.37 + 0.61 * double(circ.size());
+                        o.num_params = 1;
+                        circ.push_back(o);
+                    }
+                for (int s0 = 0; s0 < 2; ++s0)
+                    for (int w = s0; w + 1 < n; w += 2) {
+                        wire_store.push_back({w, w + 1});
+                        lgh_op o{};
+                        o.gate = LGH_GATE_CZ;
+                        o.num_wires = 2;
+                        o.wires = wire_store.back().data();
+                        circ.push_back(o);
+                    }
+            }
+        }
+        for (const lgh_op &o : circ) validate_op(o, n);
+        int64_t npar = 0;
+        for (const lgh_op &o : circ) npar += o.num_params > 0;
+        LGH_REQUIRE(npar > 0, "the circuit has no parametrised gate to differentiate");
+        std::vector<int64_t> tp(static_cast<size_t>(npar));
+        for (int64_t k = 0; k < npar; ++k) tp[size_t(k)] = k;
+        std::vector<double> jac(static_cast<size_t>(npar));
+        DeviceGuard g(device);
+        Owned psi, bra;
+        ok(lgh_sv_create(&psi.sv, n, dtype, device, nullptr));
+        ok(lgh_sv_create(&bra.sv, n, dtype, device, nullptr));
+        hipStream_t st = psi.sv->ctx->stream;
+        const int32_t zw = 0;
+        const int64_t ncirc = int64_t(circ.size());
+
+        const FusedCfg fwd_c[] = {{3, 512}, {3, 4096}, {4, 512}, {4, 4096}};
+        const FusedCfg adj_c[] = {{3, 128}, {3, 256}, {3, 512}, {4, 128}, {4, 256}, {4, 512}};
+        const FusedCfg def_f = fused_cfg(device, n, dtype, 0), def_a = fused_cfg(device, n, dtype, 1);
+        FusedCfg best_f = def_f, best_a = def_a;
+        double tbest_f = 1e300, tbest_a = 1e300;
+        ok(lgh_sv_reset(psi.sv));
+        for (const FusedCfg &c : fwd_c) {
+            ForceCfg fc(0, &c);
+            const double ms = time_on(st, 2, [&] { ok(lgh_apply(psi.sv, circ.data(), ncirc)); });
+            const int q = out->candidates++;
+            out->cand_dir[q] = 0;
+            out->cand_tile_bits[q] = c.rb;
+            out->cand_waves_per_cu[q] = c.waves_per_cu;
+            out->cand_ms[q] = ms;
+            if (c.rb == def_f.rb && c.waves_per_cu == def_f.waves_per_cu) out->default_forward_ms = ms;
+            if (ms < tbest_f) {
+                tbest_f = ms;
+                best_f = c;
+            }
+        }
+        // adjoint of <Z_0> on the circuit's state; the observable state is
+        // rebuilt before every timed sweep (the sweep overwrites it)
+        ok(lgh_sv_reset(psi.sv));
+        ok(lgh_apply(psi.sv, circ.data(), ncirc));
+        for (const FusedCfg &c : adj_c) {
+            ForceCfg fc(1, &c);
+            double ms = 1e300;
+            for (int r = 0; r < 3; ++r) {
+                ok(lgh_sv_pauli_word_from(bra.sv, psi.sv, "Z", &zw, 1));
+                ok(lgh_sv_synchronize(bra.sv));
+                const auto t0 = std::chrono::steady_clock::now();
+                lgh_sv *bl[1] = {bra.sv};
+                ok(lgh_adjoint_jacobian(psi.sv, bl, 1, circ.data(), ncirc, tp.data(), npar, 0, jac.data()));
+                const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+                if (r > 0) ms = std::min(ms, t); // the first sweep warms up
+            }
+            const int q = out->candidates++;
+            out->cand_dir[q] = 1;
+            out->cand_tile_bits[q] = c.rb;
+            out->cand_waves_per_cu[q] = c.waves_per_cu;
+            out->cand_ms[q] = ms;
+            if (c.rb == def_a.rb && c.waves_per_cu == def_a.waves_per_cu) out->default_adjoint_ms = ms;
+            if (ms < tbest_a) {
+                tbest_a = ms;
+                best_a = c;
+            }
+        }
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tuned[std::make_tuple(device, n, int(dtype), 0)] = best_f;
+            g_tuned[std::make_tuple(device, n, int(dtype), 1)] = best_a;
+        }
+        out->forward_tile_bits = best_f.rb;
+        out->forward_waves_per_cu = best_f.waves_per_cu;
+        out->adjoint_tile_bits = best_a.rb;
+        out->adjoint_waves_per_cu = best_a.waves_per_cu;
+        out->forward_ms = tbest_f;
+        out->adjoint_ms = tbest_a;
+    });
+}
+
+extern "C" lgh_status lgh_autotune_clear(void) {
+    return guard([&] { clear_tuned(); });
 }

@@ -56,6 +56,15 @@ class Op(C.Structure):
     ]
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("forward_tile_bits", C.c_int32), ("forward_waves_per_cu", C.c_int32),
+                ("adjoint_tile_bits", C.c_int32), ("adjoint_waves_per_cu", C.c_int32),
+                ("forward_ms", C.c_double), ("adjoint_ms", C.c_double),
+                ("default_forward_ms", C.c_double), ("default_adjoint_ms", C.c_double),
+                ("candidates", C.c_int32), ("cand_dir", C.c_int32 * 16), ("cand_tile_bits", C.c_int32 * 16),
+                ("cand_waves_per_cu", C.c_int32 * 16), ("cand_ms", C.c_double * 16)]
+
+
 _lib = None
 
 
@@ -115,6 +124,7 @@ def lib():
         L.lgh_stream_synchronize.argtypes = [vp]
         L.lgh_set_tile_bits.argtypes = [C.c_int, C.c_int]
         L.lgh_set_pass_waves.argtypes = [C.c_int64, C.c_int64]
+        L.lgh_autotune_fused.argtypes = [C.c_int, C.c_int32, C.c_int, vp, C.c_int64, C.POINTER(TuneResult)]
         L.lgh_fusion_plan.argtypes = [C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int] + [C.POINTER(C.c_int32)] * 4
         L.lgh_profile_enable.argtypes = [C.c_int]
         L.lgh_profile_read.argtypes = [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
@@ -122,6 +132,7 @@ def lib():
                                       C.c_int32]
         L.lgh_tmpl_destroy.argtypes = [vp]
         L.lgh_tmpl_info.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+        L.lgh_tmpl_passes.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
         L.lgh_batch_run.argtypes = [vp, C.c_int32, C.c_int, vp, vp, vp, vp, vp, vp]
         L.lgh_batch_run_serial.argtypes = [vp, C.c_int32, C.c_int, vp, vp, vp, vp, vp, vp]
         L.lgh_batch_plan_create.argtypes = [C.POINTER(vp), vp, C.c_int32, C.c_int, C.c_int32]
@@ -159,6 +170,32 @@ def set_pass_waves(forward: int, adjoint: int):
     """Caps on the wavefronts of the fused passes' grids (0: the defaults);
     each wavefront then steps through a run of consecutive tiles."""
     check(lib().lgh_set_pass_waves(int(forward), int(adjoint)))
+
+
+def autotune_fused(num_qubits, ops=None, dtype=np.complex128, device=0):
+    """Times the fused passes of ``ops`` (default: 2 layers RY/RZ + CZ brick)
+    on ``num_qubits`` qubits under every candidate tile height and grid and
+    keeps the fastest forward and adjoint choice for that size and dtype
+    (lgh_autotune_fused).  Returns the choice and every candidate's time."""
+    r = TuneResult()
+    code = LGH_C128 if np.dtype(dtype) == np.complex128 else LGH_C64
+    if ops:
+        arr, keep = make_ops(ops)
+        check(lib().lgh_autotune_fused(int(device), int(num_qubits), code, arr, len(ops), C.byref(r)))
+    else:
+        check(lib().lgh_autotune_fused(int(device), int(num_qubits), code, None, 0, C.byref(r)))
+    cands = [{"dir": "forward" if r.cand_dir[q] == 0 else "adjoint", "tile_bits": r.cand_tile_bits[q],
+              "waves_per_cu": r.cand_waves_per_cu[q], "ms": round(r.cand_ms[q], 4)} for q in range(r.candidates)]
+    return {"forward": {"tile_bits": r.forward_tile_bits, "waves_per_cu": r.forward_waves_per_cu,
+                        "ms": round(r.forward_ms, 4), "default_ms": round(r.default_forward_ms, 4)},
+            "adjoint": {"tile_bits": r.adjoint_tile_bits, "waves_per_cu": r.adjoint_waves_per_cu,
+                        "ms": round(r.adjoint_ms, 4), "default_ms": round(r.default_adjoint_ms, 4)},
+            "candidates": cands}
+
+
+def autotune_clear():
+    """Drops every tuned choice (the defaults apply again)."""
+    check(lib().lgh_autotune_clear())
 
 
 def fusion_plan(n, ops, adjoint=False, dtype=np.complex128):
@@ -471,6 +508,12 @@ class Template:
         check(lib().lgh_tmpl_info(self.handle, C.byref(npar), C.byref(nq)))
         self.num_params, self.num_qubits = npar.value, nq.value
         self._plans = weakref.WeakSet()  # batch plans that use this template
+
+    def passes(self):
+        """(forward passes, reverse passes, qubits of the states) of a batched run."""
+        f, r, q = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().lgh_tmpl_passes(self.handle, C.byref(f), C.byref(r), C.byref(q)))
+        return f.value, r.value, q.value
 
     def __del__(self):
         try:

@@ -203,3 +203,33 @@ def test_batch_plan_without_gradient():
     ev_e, _ = N.batch_run([t] * 5, thetas, grad=False)
     np.testing.assert_array_equal(ev, ev_e)
     plan.close()
+
+
+def test_batch_15_to_20_qubits_match_oracle():
+    """Circuits at the top of the batched workload's width range (15-20
+    qubits) in one batch, complex128, against the oracle."""
+    rng = np.random.default_rng(1520)
+    widths = [20, 15, 17, 19]
+    tm = {w: N.Template(w, *ring_template(w), [("Z", 0)]) for w in widths}
+    ts = [tm[w] for w in widths]
+    thetas = [rng.uniform(0, 2 * np.pi, t.num_params) for t in ts]
+    ev, jac = N.batch_run(ts, thetas)
+    for c, w in enumerate(widths):
+        e, g = oracle_run(w, *ring_template(w), thetas[c], [("Z", 0)])
+        close128(ev[c], e)
+        close128(jac[c], g)
+
+
+def test_batch_c64_multi_circuit_matches_oracle():
+    """A complex64 batch of several circuits of different widths against the
+    complex128 oracle (within 1e-5)."""
+    rng = np.random.default_rng(6464)
+    widths = [9, 14, 11, 12, 10, 13, 16]
+    tm = {w: N.Template(w, *ring_template(w), [("Z", 0)]) for w in set(widths)}
+    ts = [tm[w] for w in widths]
+    thetas = [rng.uniform(0, 2 * np.pi, t.num_params) for t in ts]
+    ev, jac = N.batch_run(ts, thetas, dtype=np.complex64)
+    for c, w in enumerate(widths):
+        e, g = oracle_run(w, *ring_template(w), thetas[c], [("Z", 0)])
+        close64(ev[c], e)
+        close64(jac[c], g)

@@ -681,3 +681,41 @@ def test_fused_deep_single_pass_rotation_scale(dtype):
                              [N.State(n, dtype).upload(b.astype(dtype)) for b in bras], ops, tp)
     assert np.all(np.isfinite(got))
     close(got, ref)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_autotuned_fused_passes_match_oracle(dtype):
+    """lgh_autotune_fused times every candidate tile height and grid on a
+    16-qubit circuit and keeps the fastest; forward states and adjoint
+    Jacobians computed under the tuned choice still match the oracle."""
+    n = 16
+    rng = np.random.default_rng(1616)
+    ops = []
+    for _ in range(3):
+        for w in range(n):
+            ops.append(("RY", [w], [float(rng.uniform(-3, 3))], False))
+            ops.append(("RZ", [w], [float(rng.uniform(-3, 3))], bool(w % 3 == 0)))
+        for s0 in (0, 1):
+            for w in range(s0, n - 1, 2):
+                ops.append(("CZ", [w, w + 1], [], False))
+    try:
+        res = N.autotune_fused(n, ops, dtype=dtype)
+        assert res["forward"]["tile_bits"] in (3, 4) and res["adjoint"]["tile_bits"] in (3, 4)
+        assert res["forward"]["waves_per_cu"] in (512, 4096)
+        assert res["adjoint"]["waves_per_cu"] in (128, 256, 512)
+        assert len(res["candidates"]) == 10
+        assert res["forward"]["ms"] <= res["forward"]["default_ms"] + 1e-9
+        assert res["adjoint"]["ms"] <= res["adjoint"]["default_ms"] + 1e-9
+        psi = N.State(n, dtype).reset().apply(ops)
+        ref = po.apply_ops(n, po.zero_state(n), ops)
+        (close128 if dtype == np.complex128 else close64)(psi.download(), ref)
+        bra = N.State(n, dtype)
+        bra.copy_from(psi)
+        bra.apply([("PauliZ", [0], [], False)])
+        npar = sum(1 for o in ops if o[2])
+        got = N.adjoint_jacobian(psi, [bra], ops, list(range(npar)))
+        jref = po.adjoint_jacobian(n, ref, [po.apply_ops(n, ref, [("PauliZ", [0], [], False)])], ops,
+                                   list(range(npar)))
+        (close128 if dtype == np.complex128 else close64)(got, jref)
+    finally:
+        N.autotune_clear()
