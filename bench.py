@@ -228,14 +228,13 @@ def bench_measurements(N, args):
     words, wires, coeffs = random_pauli_hamiltonian(n, 256, seed=args.seed)
     psi.expval_pauli_words(words, wires, coeffs)  # warm
     ms = sorted(timer.time(lambda: psi.expval_pauli_words(words, wires, coeffs)) for _ in range(5))[2]
-    # one pass over the state per group of at most 8 words sharing an X/Y mask
-    masks = {}
-    for wd, ws in zip(words, wires):
-        m = sum(1 << (n - 1 - w) for p, w in zip(wd, ws) if p in "XY")
-        masks[m] = masks.get(m, 0) + 1
-    passes = sum((c + 7) // 8 for c in masks.values())
-    out["pauli_words"] = {"qubits": n, "words": len(words), "ms": round(ms, 3), "passes": passes,
-                          "bytes": passes * S, "GBps": round(passes * S / (ms * 1e-3) / 1e9, 1),
+    # one read of the state per launch: tile passes of up to 32 words, pair
+    # passes of up to 8 words sharing a flip mask
+    plan = N.pauli_plan(n, words, wires)
+    passes = plan["tile_launches"] + plan["pair_launches"]
+    out["pauli_words"] = {"qubits": n, "words": len(words), "ms": round(ms, 3), "plan": plan,
+                          "state_reads": passes, "bytes": passes * S,
+                          "GBps": round(passes * S / (ms * 1e-3) / 1e9, 1),
                           "words_per_s": round(len(words) / (ms * 1e-3), 1)}
     # dense expectation <Z0 Z23> and a 3-wire Hermitian (one read pass each), and
     # the variance of the latter as the reference computes it (two passes)

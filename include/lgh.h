@@ -243,6 +243,15 @@ lgh_status lgh_expval_matrix(const lgh_sv *sv, const int32_t *wires, int32_t num
 lgh_status lgh_expval_pauli_words(const lgh_sv *sv, int64_t num_words, const char *paulis,
                                   const int32_t *wires, const int64_t *offsets,
                                   const double *coeffs, double *per_word, double *result);
+/* Host-only plan of lgh_expval_pauli_words (no device needed): tile passes
+ * (words grouped by the index bits their flips touch; each pass reads the
+ * state once per launch of up to 32 words) and the pair-kernel launches of
+ * words whose flips span more bits than a tile holds (one pass per flip
+ * mask, up to 8 words per launch).  No counterpart in the reference; for
+ * byte accounting and tests. */
+lgh_status lgh_pauli_plan(int32_t num_qubits, int64_t num_words, const char *paulis, const int32_t *wires,
+                          const int64_t *offsets, int32_t *tile_passes, int32_t *tile_launches,
+                          int32_t *pair_launches);
 /* lgh_expval_pauli_words without any allocation: no counterpart in the
  * reference.  For callers that cannot allocate device memory (code under HIP
  * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Same

@@ -243,17 +243,133 @@ __global__ __launch_bounds__(kBlock) void k_expval_lds(const cx<R> *__restrict__
 }
 
 // ---------------------------------------------------------------- Pauli words
+//
+// <P> for a word with X/Y bits f (flip mask), Z/Y bits zy and nY factors Y:
+// P|i> = i^nY (-1)^popcount(i & zy) |i ^ f>, so
+//   <P> = Re( i^nY sum_i (-1)^popcount(i & zy) conj(psi_{i^f}) psi_i ),
+// real for a Hermitian word.  Kernels accumulate the term's real and
+// imaginary parts and keep the one i^nY turns into the real part (`phase` =
+// nY mod 4: +re, -im, -re, +im).
+//
+// Words are grouped into passes by tiles: a pass holds a set of index bits
+// (the three lowest, so that loads cover 128-byte runs, and up to eight more
+// chosen per pass); a workgroup stages each tile of 2^11 amplitudes (all
+// combinations of the pass's bits for one value of the others) in LDS, and
+// every word of the pass whose flip bits are tile bits finds each partner
+// i ^ f in the tile.  One read of the state then serves every word of the
+// pass, whatever their flip masks.  Words with more flip bits than a tile
+// holds take the pair kernel below, one pass per flip mask.
 
-constexpr int kMaxWordsPerPass = 8;
+constexpr int kPTileBits = 11;       // amplitudes per tile: 2^11
+constexpr int kPFixedBits = 3;       // lowest index bits in every tile
+constexpr int kPassWords = 32;       // words per tile launch
+constexpr int kMaxWordsPerPass = 8;  // words per pair-kernel launch
+
+// a pass's words as kernel arguments
+struct PWords {
+    int count;
+    uint32_t flip[kPassWords];  // flip mask in tile-local bits
+    uint32_t zyl[kPassWords];   // Z/Y mask in tile-local bits
+    uint32_t phase[kPassWords]; // nY mod 4
+    uint64_t zy[kPassWords];    // Z/Y mask over global index bits (only those outside the tile are read)
+};
+// tile-local bit r is global index bit bit[r]
+struct PTile {
+    int g;
+    int bit[kPTileBits];
+};
+
+// Sign flip of a double by an integer xor of its high word (bit 31 of
+// `neg`): one instruction instead of a multiply by -1.
+__device__ __forceinline__ double flip_sign(double x, uint32_t neg) {
+    const long long b = __double_as_longlong(x) ^ (static_cast<long long>(neg & 0x80000000u) << 32);
+    return __longlong_as_double(b);
+}
+
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_pauli_tile(const cx<R> *__restrict__ sv, uint64_t ntiles,
+                                                       uint64_t tiles_per_block, Fixed tfx, PTile tm, PWords pw,
+                                                       double *__restrict__ partials) {
+    constexpr int kAmps = 1 << kPTileBits;
+    constexpr int kPer = kAmps / kBlock;
+    __shared__ cx<R> tile[kAmps];
+    const int tid = threadIdx.x;
+    const int tsz = 1 << tm.g;
+    uint64_t off[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int j = tid + k * kBlock;
+        uint64_t o = 0;
+#pragma unroll
+        for (int r = 0; r < kPTileBits; ++r)
+            if (r < tm.g && ((j >> r) & 1)) o |= 1ull << tm.bit[r];
+        off[k] = o;
+    }
+    double acc[kPassWords];
+#pragma unroll
+    for (int q = 0; q < kPassWords; ++q) acc[q] = 0.0;
+    const uint64_t t0 = uint64_t(blockIdx.x) * tiles_per_block, t1 = min(t0 + tiles_per_block, ntiles);
+    for (uint64_t t = t0; t < t1; ++t) {
+        const uint64_t base = expand(t, tfx);
+        cx<R> v[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int j = tid + k * kBlock;
+            v[k] = cx<R>{R(0), R(0)};
+            if (j < tsz) {
+                v[k] = ldnt(sv + (base | off[k]));
+                tile[j] = v[k];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < kPassWords; ++q) {
+            if (q >= pw.count) break;
+            const uint32_t f = pw.flip[q], zl = pw.zyl[q], ph = pw.phase[q];
+            // the word's sign outside the tile is uniform over the tile
+            const uint32_t pbase = uint32_t(__popcll(base & pw.zy[q]) & 1) << 31;
+            // only the component i^nY turns into the real part is summed
+            double sum = 0.0;
+            if ((ph & 1u) == 0) {
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    const int j = tid + k * kBlock;
+                    if (j < tsz) {
+                        const cx<R> p = tile[j ^ int(f)];
+                        const double x = double(p.re) * v[k].re + double(p.im) * v[k].im;
+                        sum += flip_sign(x, (uint32_t(__popc(uint32_t(j) & zl)) << 31) ^ pbase);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    const int j = tid + k * kBlock;
+                    if (j < tsz) {
+                        const cx<R> p = tile[j ^ int(f)];
+                        const double x = double(p.re) * v[k].im - double(p.im) * v[k].re;
+                        sum += flip_sign(x, (uint32_t(__popc(uint32_t(j) & zl)) << 31) ^ pbase);
+                    }
+                }
+            }
+            // i^nY: +re, -im, -re, +im
+            acc[q] += (ph == 1u || ph == 2u) ? -sum : sum;
+        }
+        __syncthreads(); // the next tile overwrites the LDS copy
+    }
+    block_sum<kPassWords>(acc);
+    if (tid == 0)
+        for (int q = 0; q < pw.count; ++q) partials[size_t(q) * gridDim.x + blockIdx.x] = acc[q];
+}
+
 struct WordSet {
     int count;
     uint64_t zy[kMaxWordsPerPass];
+    uint32_t phase[kMaxWordsPerPass];
 };
 
 // <P_w> for words that share one flip mask f.  With f != 0 each pair
 // (i, i^f) is read once: its two terms s_i conj(psi_j) psi_i and
-// s_j conj(psi_i) psi_j are added together.  The phase i^{nY} is applied on
-// the host.
+// s_j conj(psi_i) psi_j are added together.
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_pauli(const cx<R> *__restrict__ sv, uint64_t count, uint64_t chunk,
                                                   int pivot, uint64_t flip, WordSet ws,
@@ -288,9 +404,16 @@ __global__ __launch_bounds__(kBlock) void k_pauli(const cx<R> *__restrict__ sv, 
             }
         }
     }
-    block_sum<2 * kMaxWordsPerPass>(acc);
+    double val[kMaxWordsPerPass];
+#pragma unroll
+    for (int w = 0; w < kMaxWordsPerPass; ++w) {
+        const uint32_t ph = w < ws.count ? ws.phase[w] : 0u;
+        const double re = acc[2 * w], im = acc[2 * w + 1];
+        val[w] = ph == 0 ? re : ph == 1 ? -im : ph == 2 ? -re : im;
+    }
+    block_sum<kMaxWordsPerPass>(val);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 2 * ws.count; ++k) partials[size_t(k) * gridDim.x + blockIdx.x] = acc[k];
+        for (int w = 0; w < ws.count; ++w) partials[size_t(w) * gridDim.x + blockIdx.x] = val[w];
 }
 
 // ---------------------------------------------------------------- CSR
@@ -612,16 +735,6 @@ template <class R> void expval_t(const lgh_sv *sv, const int32_t *wires, int k, 
     *im = out[1];
 }
 
-// <P> = i^{nY} (re + i im) for the word's accumulated (re, im): its real part
-double word_value(double re, double im, int ny) {
-    switch (ny & 3) {
-    case 0: return re;
-    case 1: return -im;
-    case 2: return -re;
-    default: return im;
-    }
-}
-
 // sum_w Re(c_w <P_w>) with <P_w> real, in word order; per_word (nullable) gets <P_w>
 double weighted_total(int64_t nwords, const double *coeffs, const double *vals, double *per_word) {
     double tot = 0.0;
@@ -632,84 +745,220 @@ double weighted_total(int64_t nwords, const double *coeffs, const double *vals, 
     return tot;
 }
 
+// Parses word w into its flip mask, Z/Y mask and number of Y factors.
+void parse_word(int n, const char *paulis, const int32_t *wires, int64_t b, int64_t e, uint64_t &flip,
+                uint64_t &zy, int32_t &ny) {
+    flip = zy = 0;
+    ny = 0;
+    for (int64_t j = b; j < e; ++j) {
+        const int wire = wires[j];
+        LGH_REQUIRE(wire >= 0 && wire < n, "Pauli word wire out of range");
+        const uint64_t bit = 1ull << (n - 1 - wire);
+        switch (paulis[j]) {
+        case 'I': break;
+        case 'X': flip |= bit; break;
+        case 'Y': flip |= bit; zy |= bit; ny++; break;
+        case 'Z': zy |= bit; break;
+        default: LGH_REQUIRE(false, "unknown Pauli factor (words use I, X, Y, Z)");
+        }
+    }
+}
+
+// Tile geometry of the Pauli passes on an n-qubit state.
+struct PGeo {
+    int g;          // tile bits
+    int fixed;      // lowest index bits in every tile
+    int chosen_max; // bits chosen per pass
+};
+PGeo pauli_geo(int n) {
+    PGeo p;
+    p.g = std::min(kPTileBits, n);
+    p.fixed = std::min(kPFixedBits, n);
+    p.chosen_max = p.g - p.fixed;
+    return p;
+}
+
+// Groups words into tile passes without allocating: pass_of[w] gets the pass
+// of word w (-1: its flip bits do not fit a tile), pass_mask[p] the chosen
+// bits of pass p; returns the number of passes.  Each pass is filled first
+// with the uncovered words of most flip bits that still fit the pass's
+// budget of chosen bits (first fit, decreasing), then takes every
+// uncovered word whose flip bits lie in the tile.
+int64_t plan_pauli_tiles(int n, int64_t nwords, const uint64_t *flip, int32_t *pass_of, uint64_t *pass_mask) {
+    const PGeo geo = pauli_geo(n);
+    const uint64_t fixed_mask = (1ull << geo.fixed) - 1ull;
+    int64_t left = 0;
+    for (int64_t w = 0; w < nwords; ++w) {
+        const int c = __builtin_popcountll(flip[w] & ~fixed_mask);
+        pass_of[w] = c <= geo.chosen_max ? -2 : -1; // -2: still to place
+        left += pass_of[w] == -2;
+    }
+    int64_t np = 0;
+    while (left > 0) {
+        uint64_t chosen = 0;
+        for (int c = geo.chosen_max; c >= 0; --c)
+            for (int64_t w = 0; w < nwords; ++w) {
+                if (pass_of[w] != -2) continue;
+                const uint64_t rest = flip[w] & ~fixed_mask;
+                if (__builtin_popcountll(rest) == c && __builtin_popcountll(chosen | rest) <= geo.chosen_max)
+                    chosen |= rest;
+            }
+        for (int b = geo.fixed; b < n && __builtin_popcountll(chosen) < geo.chosen_max; ++b) chosen |= 1ull << b;
+        for (int64_t w = 0; w < nwords; ++w)
+            if (pass_of[w] == -2 && ((flip[w] & ~fixed_mask) & ~chosen) == 0) {
+                pass_of[w] = int32_t(np);
+                --left;
+            }
+        pass_mask[np++] = chosen;
+    }
+    return np;
+}
+
+struct PauliBufs {
+    double *partials; // device, partials_cap doubles
+    size_t partials_cap;
+    double *results;  // device, one double per word
+};
+
+// Device doubles of reduction partials the Pauli passes of an n-qubit state need.
+size_t pauli_partials_doubles(int n, int device) {
+    const uint64_t len = 1ull << n;
+    const unsigned pair_blocks = chunks_for(len, device).blocks;
+    const unsigned tile_blocks = unsigned(std::max(1, sm_count(device))) * 4;
+    return std::max(size_t(kMaxWordsPerPass) * pair_blocks, size_t(kPassWords) * tile_blocks);
+}
+
+// Launches every pass of the words (tile passes first, in pass order, then
+// pair passes in ascending flip mask) and their reductions into
+// bufs.results; slot_word[s] is the word whose value lands in result slot s.
+// Needs no allocation: the caller provides every array (nwords entries each).
+template <class R>
+void pauli_run(const lgh_sv *sv, int64_t nwords, const uint64_t *flip, const uint64_t *zy, const int32_t *ny,
+               int32_t *pass_of, uint64_t *pass_mask, bool *done, int32_t *slot_word, const PauliBufs &bufs) {
+    StreamCtx *ctx = sv->ctx;
+    const int n = sv->n;
+    const uint64_t len = sv->length();
+    const PGeo geo = pauli_geo(n);
+    const int64_t npasses = plan_pauli_tiles(n, nwords, flip, pass_of, pass_mask);
+    int64_t nslots = 0;
+    // tile passes
+    const uint64_t ntiles = len >> geo.g;
+    const unsigned max_blocks = unsigned(std::max(1, sm_count(sv->device))) * 4;
+    const uint64_t per_block = (ntiles + max_blocks - 1) / max_blocks;
+    const unsigned tblocks = unsigned((ntiles + per_block - 1) / per_block);
+    LGH_REQUIRE(size_t(kPassWords) * tblocks <= bufs.partials_cap, "Pauli partials buffer too small");
+    for (int64_t p = 0; p < npasses; ++p) {
+        PTile tm{};
+        tm.g = geo.g;
+        int r = 0;
+        for (int b = 0; b < geo.fixed; ++b) tm.bit[r++] = b;
+        for (int b = geo.fixed; b < n; ++b)
+            if ((pass_mask[p] >> b) & 1) tm.bit[r++] = b;
+        LGH_REQUIRE(r == geo.g, "Pauli tile: bad bit count");
+        Fixed tfx{};
+        tfx.cnt = geo.g;
+        for (int q = 0; q < geo.g; ++q) tfx.pos[q] = tm.bit[q]; // ascending
+        PWords pw{};
+        auto launch = [&] {
+            hipLaunchKernelGGL(k_pauli_tile<R>, dim3(tblocks), dim3(kBlock), 0, ctx->stream, cdata<R>(sv), ntiles,
+                               per_block, tfx, tm, pw, bufs.partials);
+            LGH_LAUNCH_CHECK();
+            launch_reduce(ctx, bufs.partials, int(tblocks), pw.count, bufs.results + (nslots - pw.count));
+            pw.count = 0;
+        };
+        for (int64_t w = 0; w < nwords; ++w) {
+            if (pass_of[w] != p) continue;
+            uint32_t fl = 0, zl = 0;
+            uint64_t tile_mask = 0;
+            for (int q = 0; q < geo.g; ++q) {
+                if ((flip[w] >> tm.bit[q]) & 1) fl |= 1u << q;
+                if ((zy[w] >> tm.bit[q]) & 1) zl |= 1u << q;
+                tile_mask |= 1ull << tm.bit[q];
+            }
+            pw.flip[pw.count] = fl;
+            pw.zyl[pw.count] = zl;
+            pw.zy[pw.count] = zy[w] & ~tile_mask;
+            pw.phase[pw.count] = uint32_t(ny[w] & 3);
+            ++pw.count;
+            slot_word[nslots++] = int32_t(w);
+            if (pw.count == kPassWords) launch();
+        }
+        if (pw.count > 0) launch();
+    }
+    // words wider than a tile: one pair pass per flip mask, ascending
+    for (int64_t w = 0; w < nwords; ++w) done[w] = pass_of[w] >= 0;
+    for (;;) {
+        bool any = false;
+        uint64_t f = 0;
+        for (int64_t w = 0; w < nwords; ++w)
+            if (!done[w] && (!any || flip[w] < f)) {
+                f = flip[w];
+                any = true;
+            }
+        if (!any) break;
+        const int pivot = f ? __builtin_ctzll(f) : -1;
+        const uint64_t count = f ? len / 2 : len;
+        const Chunks ch = chunks_for(count, sv->device);
+        LGH_REQUIRE(size_t(kMaxWordsPerPass) * ch.blocks <= bufs.partials_cap, "Pauli partials buffer too small");
+        WordSet ws{};
+        auto launch = [&] {
+            hipLaunchKernelGGL(k_pauli<R>, dim3(ch.blocks), dim3(kBlock), 0, ctx->stream, cdata<R>(sv), count,
+                               ch.chunk, pivot, f, ws, bufs.partials);
+            LGH_LAUNCH_CHECK();
+            launch_reduce(ctx, bufs.partials, ch.blocks, ws.count, bufs.results + (nslots - ws.count));
+            ws.count = 0;
+        };
+        for (int64_t w = 0; w < nwords; ++w) {
+            if (done[w] || flip[w] != f) continue;
+            done[w] = true;
+            ws.zy[ws.count] = zy[w];
+            ws.phase[ws.count] = uint32_t(ny[w] & 3);
+            ++ws.count;
+            slot_word[nslots++] = int32_t(w);
+            if (ws.count == kMaxWordsPerPass) launch();
+        }
+        if (ws.count > 0) launch();
+    }
+    LGH_REQUIRE(nslots == nwords, "Pauli words: every word gets one result slot");
+}
+
 template <class R>
 void pauli_t(const lgh_sv *sv, int64_t nwords, const char *paulis, const int32_t *wires, const int64_t *offsets,
              const double *coeffs, double *per_word, double *result) {
     StreamCtx *ctx = sv->ctx;
-    const int n = sv->n;
-    struct W {
-        uint64_t flip = 0, zy = 0;
-        int ny = 0;
-    };
-    std::vector<W> words(static_cast<size_t>(nwords));
-    std::map<uint64_t, std::vector<int64_t>> groups;
-    for (int64_t w = 0; w < nwords; ++w) {
-        W &x = words[size_t(w)];
-        for (int64_t j = offsets[w]; j < offsets[w + 1]; ++j) {
-            const int wire = wires[j];
-            LGH_REQUIRE(wire >= 0 && wire < n, "Pauli word wire out of range");
-            const uint64_t bit = 1ull << (n - 1 - wire);
-            switch (paulis[j]) {
-            case 'I': break;
-            case 'X': x.flip |= bit; break;
-            case 'Y': x.flip |= bit; x.zy |= bit; x.ny++; break;
-            case 'Z': x.zy |= bit; break;
-            default: throw Error(LGH_ERR_INVALID_ARGUMENT, std::string("unknown Pauli factor '") + paulis[j] + "'");
-            }
-        }
-        groups[x.flip].push_back(w);
-    }
-    std::vector<double> vals(size_t(nwords), 0.0);
-    const uint64_t len = sv->length();
-    // every pass reduces into its own result slots; one copy to the host at the end
-    const unsigned max_blocks = std::max(chunks_for(len, sv->device).blocks, chunks_for(len / 2, sv->device).blocks);
-    const size_t part_doubles = size_t(2 * kMaxWordsPerPass) * max_blocks;
-    char *ws_base = static_cast<char *>(ctx->workspace(sizeof(double) * (part_doubles + 2 * size_t(nwords)) + 256));
-    double *partials = reinterpret_cast<double *>(ws_base);
-    double *results = partials + part_doubles;
-    std::vector<int64_t> slot_word; // word of each result slot pair
-    for (auto &kv : groups) {
-        const uint64_t flip = kv.first;
-        const int pivot = flip ? __builtin_ctzll(flip) : -1;
-        const uint64_t count = flip ? len / 2 : len;
-        Chunks ch = chunks_for(count, sv->device);
-        const auto &ids = kv.second;
-        for (size_t b0 = 0; b0 < ids.size(); b0 += kMaxWordsPerPass) {
-            WordSet ws{};
-            ws.count = int(std::min<size_t>(kMaxWordsPerPass, ids.size() - b0));
-            for (int q = 0; q < ws.count; ++q) {
-                ws.zy[q] = words[size_t(ids[b0 + q])].zy;
-                slot_word.push_back(ids[b0 + q]);
-            }
-            hipLaunchKernelGGL(k_pauli<R>, dim3(ch.blocks), dim3(kBlock), 0, ctx->stream, cdata<R>(sv), count,
-                               ch.chunk, pivot, flip, ws, partials);
-            LGH_LAUNCH_CHECK();
-            launch_reduce(ctx, partials, ch.blocks, 2 * ws.count,
-                          results + 2 * (slot_word.size() - size_t(ws.count)));
-        }
-    }
-    std::vector<double> out(2 * size_t(nwords));
-    if (nwords > 0) fetch(ctx, results, int(2 * nwords), out.data());
-    for (size_t q = 0; q < slot_word.size(); ++q)
-        vals[size_t(slot_word[q])] = word_value(out[2 * q], out[2 * q + 1], words[size_t(slot_word[q])].ny);
+    const size_t nw = size_t(nwords);
+    std::vector<uint64_t> flip(nw), zy(nw), pass_mask(std::max<size_t>(1, nw));
+    std::vector<int32_t> ny(nw), pass_of(nw), slot_word(nw);
+    std::unique_ptr<bool[]> done(new bool[std::max<size_t>(1, nw)]);
+    for (int64_t w = 0; w < nwords; ++w)
+        parse_word(sv->n, paulis, wires, offsets[w], offsets[w + 1], flip[size_t(w)], zy[size_t(w)], ny[size_t(w)]);
+    const size_t pcap = pauli_partials_doubles(sv->n, sv->device);
+    char *ws_base = static_cast<char *>(ctx->workspace(sizeof(double) * (pcap + nw) + 256));
+    PauliBufs bufs{reinterpret_cast<double *>(ws_base), pcap, reinterpret_cast<double *>(ws_base) + pcap};
+    if (nwords > 0)
+        pauli_run<R>(sv, nwords, flip.data(), zy.data(), ny.data(), pass_of.data(), pass_mask.data(), done.get(),
+                     slot_word.data(), bufs);
+    std::vector<double> out(std::max<size_t>(1, nw)), vals(std::max<size_t>(1, nw));
+    if (nwords > 0) fetch(ctx, bufs.results, int(nwords), out.data());
+    for (size_t q = 0; q < nw; ++q) vals[size_t(slot_word[q])] = out[q];
     *result = weighted_total(nwords, coeffs, vals.data(), per_word);
 }
 
 // ---------------------------------------------------------------- Pauli words without allocation
 //
 // lgh_expval_pauli_words_noalloc runs the passes of pauli_t (same kernels,
-// same grouping by flip mask, same chunking, so the same results bit for bit)
-// with its reduction partials and word results in `scratch`, a device array
-// of fixed size that every call on one device shares.  The host side keeps
-// its bookkeeping in fixed-size arrays on the stack.  Nothing is allocated
-// after the first call on a device, which looks up the array's device address
-// (and the device's CU count used to size the passes).
+// same grouping, same chunking, so the same results bit for bit) with its
+// reduction partials and word results in `scratch`, a device array of fixed
+// size that every call on one device shares.  The host side keeps its
+// bookkeeping in fixed-size arrays on the stack.  Nothing is allocated after
+// the first call on a device, which looks up the array's device address (and
+// the device's CU count used to size the passes).
 //
-// Layout of `scratch` for one call: partials of one pass (2 * 8 words *
-// blocks doubles) | (re, im) of every word.  Every entry a call reads was
-// written earlier in the same call (each pass writes all its partials, the
-// reduction writes the word's results), so nothing needs clearing; two calls
-// that overlap on one device race on the array.
+// Layout of `scratch` for one call: reduction partials of one launch |
+// one result per word.  Every entry a call reads was written earlier in the
+// same call (each launch writes all its partials, each reduction its
+// words' results), so nothing needs clearing; two calls that overlap on one
+// device race on the array.
 
 constexpr size_t kNoAllocScratchBytes = size_t(512) << 10;
 constexpr int64_t kNoAllocMaxWords = 1024;
@@ -742,76 +991,26 @@ void pauli_noalloc_t(const lgh_sv *sv, int64_t nwords, const char *paulis, const
                                                 std::to_string(kNoAllocMaxWords) + " words (got " +
                                                 std::to_string(nwords) + ")");
     StreamCtx *ctx = sv->ctx;
-    const int n = sv->n;
     char *base = static_cast<char *>(scratch_on(sv->device));
-    uint64_t flip[kNoAllocMaxWords], zy[kNoAllocMaxWords];
-    int32_t ny[kNoAllocMaxWords], slot_word[kNoAllocMaxWords];
+    uint64_t flip[kNoAllocMaxWords], zy[kNoAllocMaxWords], pass_mask[kNoAllocMaxWords];
+    int32_t ny[kNoAllocMaxWords], pass_of[kNoAllocMaxWords], slot_word[kNoAllocMaxWords];
     bool done[kNoAllocMaxWords];
-    for (int64_t w = 0; w < nwords; ++w) {
-        flip[w] = zy[w] = 0;
-        ny[w] = 0;
-        done[w] = false;
-        for (int64_t j = offsets[w]; j < offsets[w + 1]; ++j) {
-            const int wire = wires[j];
-            LGH_REQUIRE(wire >= 0 && wire < n, "Pauli word wire out of range");
-            const uint64_t bit = 1ull << (n - 1 - wire);
-            switch (paulis[j]) {
-            case 'I': break;
-            case 'X': flip[w] |= bit; break;
-            case 'Y': flip[w] |= bit; zy[w] |= bit; ny[w]++; break;
-            case 'Z': zy[w] |= bit; break;
-            default: LGH_REQUIRE(false, "unknown Pauli factor (words use I, X, Y, Z)");
-            }
-        }
-    }
-    const uint64_t len = sv->length();
-    const unsigned max_blocks = std::max(chunks_for(len, sv->device).blocks, chunks_for(len / 2, sv->device).blocks);
-    const size_t part_doubles = size_t(2 * kMaxWordsPerPass) * max_blocks;
-    const size_t need = sizeof(double) * (part_doubles + 2 * size_t(nwords));
+    for (int64_t w = 0; w < nwords; ++w) parse_word(sv->n, paulis, wires, offsets[w], offsets[w + 1], flip[w], zy[w], ny[w]);
+    const size_t pcap = pauli_partials_doubles(sv->n, sv->device);
+    const size_t need = sizeof(double) * (pcap + size_t(nwords));
     LGH_REQUIRE(need <= kNoAllocScratchBytes, "lgh_expval_pauli_words_noalloc: " + std::to_string(need) +
                                                   " bytes of scratch needed, " +
                                                   std::to_string(kNoAllocScratchBytes) + " available");
-    double *partials = reinterpret_cast<double *>(base);
-    double *results = partials + part_doubles;
-    // groups in ascending flip mask, words of a group in ascending order: the
-    // passes of pauli_t
-    int64_t nslots = 0;
-    for (;;) {
-        bool any = false;
-        uint64_t f = 0;
-        for (int64_t w = 0; w < nwords; ++w)
-            if (!done[w] && (!any || flip[w] < f)) {
-                f = flip[w];
-                any = true;
-            }
-        if (!any) break;
-        const int pivot = f ? __builtin_ctzll(f) : -1;
-        const uint64_t count = f ? len / 2 : len;
-        const Chunks ch = chunks_for(count, sv->device);
-        WordSet ws{};
-        auto launch = [&] {
-            hipLaunchKernelGGL(k_pauli<R>, dim3(ch.blocks), dim3(kBlock), 0, ctx->stream, cdata<R>(sv), count,
-                               ch.chunk, pivot, f, ws, partials);
-            LGH_LAUNCH_CHECK();
-            launch_reduce(ctx, partials, ch.blocks, 2 * ws.count, results + 2 * (nslots - ws.count));
-            ws.count = 0;
-        };
-        for (int64_t w = 0; w < nwords; ++w) {
-            if (done[w] || flip[w] != f) continue;
-            done[w] = true;
-            ws.zy[ws.count++] = zy[w];
-            slot_word[nslots++] = int32_t(w);
-            if (ws.count == kMaxWordsPerPass) launch();
-        }
-        if (ws.count > 0) launch();
-    }
-    double out[2 * kNoAllocMaxWords];
+    PauliBufs bufs{reinterpret_cast<double *>(base), pcap, reinterpret_cast<double *>(base) + pcap};
+    if (nwords > 0) pauli_run<R>(sv, nwords, flip, zy, ny, pass_of, pass_mask, done, slot_word, bufs);
+    double out[kNoAllocMaxWords];
     double vals[kNoAllocMaxWords];
     if (nwords > 0) {
-        LGH_HIP(hipMemcpyAsync(out, results, sizeof(double) * 2 * size_t(nwords), hipMemcpyDeviceToHost, ctx->stream));
+        LGH_HIP(hipMemcpyAsync(out, bufs.results, sizeof(double) * size_t(nwords), hipMemcpyDeviceToHost,
+                               ctx->stream));
         ctx->sync();
     }
-    for (int64_t q = 0; q < nslots; ++q) vals[slot_word[q]] = word_value(out[2 * q], out[2 * q + 1], ny[slot_word[q]]);
+    for (int64_t q = 0; q < nwords; ++q) vals[slot_word[q]] = out[q];
     *result = weighted_total(nwords, coeffs, vals, per_word);
 }
 
@@ -1046,6 +1245,38 @@ lgh_status lgh_expval_pauli_words(const lgh_sv *sv, int64_t num_words, const cha
         by_dtype(sv, [&](auto r) {
             pauli_t<decltype(r)>(sv, num_words, paulis, wires, offsets, coeffs, per_word, result);
         });
+    });
+}
+
+lgh_status lgh_pauli_plan(int32_t num_qubits, int64_t num_words, const char *paulis, const int32_t *wires,
+                          const int64_t *offsets, int32_t *tile_passes, int32_t *tile_launches,
+                          int32_t *pair_launches) {
+    return guard([&] {
+        LGH_REQUIRE(num_qubits >= 1 && num_qubits <= 36, "num_qubits must be in [1, 36]");
+        LGH_REQUIRE(num_words >= 0 && (num_words == 0 || (paulis && offsets)), "bad word list");
+        const size_t nw = size_t(num_words);
+        std::vector<uint64_t> flip(nw), zy(nw), mask(std::max<size_t>(1, nw));
+        std::vector<int32_t> ny(nw), pass_of(nw);
+        for (int64_t w = 0; w < num_words; ++w) {
+            LGH_REQUIRE(offsets[w + 1] >= offsets[w] && (offsets[w + 1] == offsets[w] || wires), "bad Pauli word offsets");
+            parse_word(num_qubits, paulis, wires, offsets[w], offsets[w + 1], flip[size_t(w)], zy[size_t(w)],
+                       ny[size_t(w)]);
+        }
+        const int64_t np = num_words ? plan_pauli_tiles(num_qubits, num_words, flip.data(), pass_of.data(), mask.data()) : 0;
+        std::vector<int64_t> per(size_t(std::max<int64_t>(1, np)), 0);
+        std::map<uint64_t, int64_t> pair;
+        for (size_t w = 0; w < nw; ++w) {
+            if (pass_of[w] >= 0)
+                ++per[size_t(pass_of[w])];
+            else
+                ++pair[flip[w]];
+        }
+        int64_t tl = 0, pl = 0;
+        for (int64_t p = 0; p < np; ++p) tl += (per[size_t(p)] + kPassWords - 1) / kPassWords;
+        for (auto &kv : pair) pl += (kv.second + kMaxWordsPerPass - 1) / kMaxWordsPerPass;
+        if (tile_passes) *tile_passes = int32_t(np);
+        if (tile_launches) *tile_launches = int32_t(tl);
+        if (pair_launches) *pair_launches = int32_t(pl);
     });
 }
 

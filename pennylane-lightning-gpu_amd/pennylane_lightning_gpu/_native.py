@@ -102,6 +102,7 @@ def lib():
         L.lgh_scale.argtypes = [vp, C.c_double, C.c_double]
         L.lgh_expval_matrix.argtypes = [vp, vp, C.c_int32, vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.lgh_expval_pauli_words.argtypes = [vp, C.c_int64, C.c_char_p, vp, vp, vp, vp, C.POINTER(C.c_double)]
+        L.lgh_pauli_plan.argtypes = [C.c_int32, C.c_int64, C.c_char_p, vp, vp] + [C.POINTER(C.c_int32)] * 3
         L.lgh_expval_pauli_words_noalloc.argtypes = [vp, C.c_int64, C.c_char_p, vp, vp, vp, vp,
                                                      C.POINTER(C.c_double)]
         L.lgh_csr_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.c_int64, vp, vp, vp, C.c_int64]
@@ -170,6 +171,19 @@ def set_pass_waves(forward: int, adjoint: int):
     """Caps on the wavefronts of the fused passes' grids (0: the defaults);
     each wavefront then steps through a run of consecutive tiles."""
     check(lib().lgh_set_pass_waves(int(forward), int(adjoint)))
+
+
+def pauli_plan(num_qubits, words, wires):
+    """Host-only plan of a Pauli-word expectation: dict with ``tile_passes``,
+    ``tile_launches`` and ``pair_launches`` (lgh_pauli_plan)."""
+    chars = "".join(words).encode()
+    wf = np.ascontiguousarray([x for ws in wires for x in ws] or [0], dtype=np.int32)
+    offs = np.zeros(len(words) + 1, dtype=np.int64)
+    offs[1:] = np.cumsum([len(w) for w in words])
+    v = [C.c_int32() for _ in range(3)]
+    check(lib().lgh_pauli_plan(int(num_qubits), len(words), chars, wf.ctypes.data, offs.ctypes.data,
+                               *[C.byref(x) for x in v]))
+    return {"tile_passes": v[0].value, "tile_launches": v[1].value, "pair_launches": v[2].value}
 
 
 def autotune_fused(num_qubits, ops=None, dtype=np.complex128, device=0):

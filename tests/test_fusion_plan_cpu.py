@@ -71,3 +71,23 @@ def test_plan_errors():
         plan(0, [("RY", [0], [0.1], False)])
     with pytest.raises(N.LghError):
         plan(4, [("RY", [7], [0.1], False)])
+
+
+def test_pauli_plan_groups_words_by_tile_bits():
+    """The Pauli-word expectation groups words whose flip bits fit one tile
+    of 2^11 amplitudes into one pass: the 24-qubit Hamiltonian of 256 random
+    words (1-4 factors) takes a handful of state reads instead of one per
+    distinct flip mask; words flipping more bits than a tile holds fall back
+    to the pair passes, one per flip mask (host-only plan, no device)."""
+    from pennylane_lightning_gpu.workloads import random_pauli_hamiltonian
+
+    n = 24
+    words, wires, _ = random_pauli_hamiltonian(n, 256, 1337)
+    masks = {sum(1 << (n - 1 - w) for p, w in zip(wd, ws) if p in "XY") for wd, ws in zip(words, wires)}
+    plan = N.pauli_plan(n, words, wires)
+    assert plan["pair_launches"] == 0
+    assert plan["tile_launches"] <= 24 < len(masks)
+    wide = N.pauli_plan(n, ["X" * 12, "XX", "Y" * 12], [list(range(12)), [0, 23], list(range(12))])
+    assert wide == {"tile_passes": 1, "tile_launches": 1, "pair_launches": 1}
+    small = N.pauli_plan(4, ["XXXX", "YZ", "Z"], [[0, 1, 2, 3], [0, 3], [2]])
+    assert small == {"tile_passes": 1, "tile_launches": 1, "pair_launches": 0}

@@ -182,6 +182,36 @@ def test_pauli_words(dtype):
     (close128 if dtype == np.complex128 else close64)(tot, rt)
 
 
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [6, 11, 17])
+def test_pauli_words_tiles_and_wide_words(dtype, n):
+    """Tile passes (words grouped by their flip bits, every letter, phases
+    i^nY of all four kinds, Z-only words) and the pair passes of words whose
+    flips span more bits than a tile holds, on states smaller than, equal to
+    and larger than one tile, against the oracle."""
+    rng = np.random.default_rng(1000 + n)
+    psi = rand_state(n, rng)
+    words, wires = [], []
+    for _ in range(70):
+        k = int(rng.integers(1, min(n, 5) + 1))
+        words.append("".join(rng.choice(list("XYZI"), k)))
+        wires.append([int(x) for x in rng.choice(n, k, replace=False)])
+    for ny in range(4):  # i^nY of every kind, flips on low and high bits
+        words.append("Y" * (ny + 1) + "X")
+        wires.append([int(x) for x in rng.choice(n, ny + 2, replace=False)])
+    words += ["X" * n, "Y" * n, "XZ" * (n // 2), "Z" * n, ""]
+    wires += [list(range(n)), list(range(n))[::-1], list(range(2 * (n // 2))), list(range(n)), []]
+    coeffs = rng.normal(size=len(words)) + 0.3j * rng.normal(size=len(words))
+    plan = N.pauli_plan(n, words, wires)
+    if n == 17:
+        assert plan["pair_launches"] >= 1 and plan["tile_launches"] >= 1
+    s = N.State(n, dtype).upload(psi.astype(dtype))
+    tot, per = s.expval_pauli_words(words, wires, coeffs)
+    rt, rp = po.expval_pauli_words(n, psi, words, wires, coeffs)
+    (close128 if dtype == np.complex128 else close64)(per, rp)
+    (close128 if dtype == np.complex128 else close64)(tot, rt)
+
+
 def tfim_csr(n, J=1.0, h=0.7):
     import test_oracle as T
 
