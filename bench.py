@@ -351,6 +351,8 @@ def serve_stream(N, reqs, streams, batch_max):
     late = np.zeros(len(reqs))  # submission after the scheduled arrival (host side), ms
     t0 = time.perf_counter()
     for i, (ta, n, ops, grad) in enumerate(reqs):
+        if i and i % 5000 == 0:
+            log(f"[bench] serving: {i} requests submitted at {time.perf_counter() - t0:.1f} s")
         dt = ta - (time.perf_counter() - t0)
         if dt > 0:
             time.sleep(dt)
@@ -361,6 +363,8 @@ def serve_stream(N, reqs, streams, batch_max):
     for i, ((ta, n, ops, grad), tk) in enumerate(zip(reqs, tickets)):
         _, _, tm = srv.wait(tk)
         lat.setdefault(int(n), []).append((tm[0] + late[i], tm[1], tm[2] + late[i]))
+        if (i + 1) % 2000 == 0:
+            log(f"[bench] serving: {i + 1} of {len(reqs)} requests done at {time.perf_counter() - t0:.1f} s")
     t_all = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
