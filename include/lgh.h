@@ -294,6 +294,20 @@ lgh_status lgh_adjoint_jacobian(const lgh_sv *psi, lgh_sv *const *bras, int32_t 
                                 const lgh_op *ops, int64_t num_ops, const int64_t *trainable,
                                 int64_t num_trainable, int32_t apply_operations, double *jac);
 
+/* lgh_adjoint_jacobian for observables that are words of Z (and identity)
+ * factors, the common case of PennyLane's expval(qml.PauliZ(i) @ ...): the
+ * reference forms every H lambda with the observable's gates before the
+ * sweep (applyObservables, AdjointDiffGPU.hpp:228-268, ObservablesGPU.hpp:88-
+ * 128, :209-260).  Here observable j is given by zmasks[j], the
+ * amplitude-index bits (bit n-1-w for wire w) of its Z factors, and bras[j]
+ * is scratch of psi's size that the sweep fills with Z_j psi itself: in its
+ * first reverse pass when that pass is fused, which saves a read and a write
+ * of the state per observable.  Results equal lgh_adjoint_jacobian's with
+ * bras[j] = Z_j psi, bit for bit. */
+lgh_status lgh_adjoint_jacobian_zwords(const lgh_sv *psi, lgh_sv *const *bras, const uint64_t *zmasks,
+                                       int32_t num_obs, const lgh_op *ops, int64_t num_ops, const int64_t *trainable,
+                                       int64_t num_trainable, int32_t apply_operations, double *jac);
+
 /* ---- batched circuits ---------------------------------------------------
  * No counterpart in the reference, which runs one circuit per device call
  * (the closest form is the per-GPU observable batching of batch_obs,

@@ -72,7 +72,8 @@ def build_lgh(force=True, jobs=8):
         list(ex.map(_run, jobs_list))
     out = os.path.join(PKG, "liblgh.so")
     tmp = out + ".tmp"
-    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-Wl,-soname,liblgh.so", "-o", tmp, *objs])
+    _run([HIPCC, "-shared", f"--offload-arch={ARCH}", "-Wl,-soname,liblgh.so", "-o", tmp, *objs,
+          "-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, out)
     return out
 

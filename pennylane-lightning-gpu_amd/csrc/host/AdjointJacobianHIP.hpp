@@ -94,20 +94,46 @@ template <class T> class AdjointJacobianHIP {
             if (fwd.size()) check(lgh_apply(applied->handle(), fwd.data(), static_cast<int64_t>(fwd.size())));
         }
         const StateVectorHIPManaged<T> &lambda = applied ? *applied : sv;
+        // Observables that are words of Z factors need no state formed
+        // before the sweep: the sweep forms Z_j lambda in its first pass.
+        const std::size_t nq = lambda.getNumQubits();
+        std::vector<uint64_t> zmasks;
+        for (const auto &o : obs) {
+            std::string letters;
+            std::vector<std::size_t> wires;
+            if (!o->pauliWord(letters, wires)) break;
+            uint64_t m = 0;
+            bool z_only = true;
+            for (std::size_t k = 0; k < letters.size(); ++k) {
+                if (letters[k] == 'Z')
+                    m |= uint64_t(1) << (nq - 1 - wires[k]);
+                else if (letters[k] != 'I')
+                    z_only = false;
+            }
+            if (!z_only) break;
+            zmasks.push_back(m);
+        }
+        const bool z_words = !obs.empty() && zmasks.size() == obs.size();
         std::vector<StateVectorHIPManaged<T>> bras;
         bras.reserve(obs.size());
         for (const auto &o : obs) {
             bras.emplace_back(lambda.getNumQubits(), lambda.getDevTag(),
                               typename StateVectorHIPManaged<T>::Uninitialized{});
-            o->applyFrom(lambda, bras.back());
+            if (!z_words) o->applyFrom(lambda, bras.back());
         }
         std::vector<lgh_sv *> handles;
         for (auto &s : bras) handles.push_back(s.handle());
         std::vector<int64_t> tp(trainable.begin(), trainable.end());
         std::vector<double> jac(obs.size() * trainable.size(), 0.0);
-        check(lgh_adjoint_jacobian(lambda.handle(), handles.data(), static_cast<int32_t>(handles.size()), b.data(),
-                                   static_cast<int64_t>(b.size()), tp.data(), static_cast<int64_t>(tp.size()), 0,
-                                   jac.data()));
+        if (z_words)
+            check(lgh_adjoint_jacobian_zwords(lambda.handle(), handles.data(), zmasks.data(),
+                                              static_cast<int32_t>(handles.size()), b.data(),
+                                              static_cast<int64_t>(b.size()), tp.data(),
+                                              static_cast<int64_t>(tp.size()), 0, jac.data()));
+        else
+            check(lgh_adjoint_jacobian(lambda.handle(), handles.data(), static_cast<int32_t>(handles.size()),
+                                       b.data(), static_cast<int64_t>(b.size()), tp.data(),
+                                       static_cast<int64_t>(tp.size()), 0, jac.data()));
         std::vector<std::vector<T>> out(obs.size(), std::vector<T>(trainable.size()));
         for (std::size_t j = 0; j < obs.size(); ++j)
             for (std::size_t t = 0; t < trainable.size(); ++t)

@@ -392,9 +392,29 @@ void fused_step(lgh_sv *lam, cx<R> **bras_dev, int nobs, const KOp &udag, const 
         });
 }
 
+// dst = Z-word(zmask) src: the sign (-1)^popcount(i & zmask) per amplitude
+template <class R>
+__global__ __launch_bounds__(kBlock) void k_zsign(cx<R> *__restrict__ dst, const cx<R> *__restrict__ src, uint64_t len,
+                                                  uint64_t zmask) {
+    const uint64_t i = xcd_block() * kBlock + threadIdx.x;
+    if (i >= len) return;
+    const cx<R> a = ldnt(src + i);
+    stnt(dst + i, (__popcll(i & zmask) & 1) ? cx<R>{-a.re, -a.im} : a);
+}
+
+template <class R> void zsign(lgh_sv *dst, const void *src, uint64_t zmask, hipStream_t stream) {
+    const uint64_t len = dst->length();
+    hipLaunchKernelGGL(k_zsign<R>, dim3(unsigned((len + kBlock - 1) / kBlock)), dim3(kBlock), 0, stream,
+                       static_cast<cx<R> *>(dst->data), static_cast<const cx<R> *>(src), len, zmask);
+    LGH_LAUNCH_CHECK();
+}
+
+// zmasks (nullable): observable j is the Z word zmasks[j] and bras[j] is
+// scratch that the sweep fills with Z_j psi (inside its first reverse pass
+// where that pass is fused).
 template <class R>
 void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op *ops, int64_t nops,
-               const int64_t *tp, int64_t ntp, bool apply_ops, double *jac_host) {
+               const int64_t *tp, int64_t ntp, bool apply_ops, double *jac_host, const uint64_t *zmasks) {
     const int n = psi_in->n;
     StreamCtx *ctx = psi_in->ctx;
     for (int j = 0; j < nobs; ++j)
@@ -418,6 +438,9 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
     if (apply_ops)
         for (int64_t i = 0; i < nops; ++i)
             if (!is_prep(ops[i])) launch_kop(&lam, compile_op(ops[i], n, false));
+    // Z-word observable states: formed here unless the fused sweep forms them
+    if (zmasks && !fused_path)
+        for (int j = 0; j < nobs; ++j) zsign<R>(bras[j], lam.data, zmasks[j], ctx->stream);
 
     // workspace: [jac (nobs*ntp) | bra pointers | partials]
     // the widest step shape has one thread per amplitude
@@ -504,8 +527,16 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
                     LGH_LAUNCH_CHECK();
                 }
             };
-            fused_adjoint_one(&lam, from_psi ? psi_in->data : nullptr, bras[j], udag, gens, slot, fac, jrow,
-                              fpartials, standalone);
+            if (zmasks) {
+                // the sweep's starting lambda: psi itself, or lam's copy of it
+                const void *start = from_psi ? psi_in->data : lam.data;
+                BraFromZ bz{zmasks[j], [&, j, start] { zsign<R>(bras[j], start, zmasks[j], ctx->stream); }};
+                fused_adjoint_one(&lam, from_psi ? psi_in->data : nullptr, bras[j], udag, gens, slot, fac, jrow,
+                                  fpartials, standalone, &bz);
+            } else {
+                fused_adjoint_one(&lam, from_psi ? psi_in->data : nullptr, bras[j], udag, gens, slot, fac, jrow,
+                                  fpartials, standalone);
+            }
         }
         if (nobs * ntp > 0)
             LGH_HIP(hipMemcpyAsync(jac_host, jac_dev, sizeof(double) * size_t(nobs * ntp), hipMemcpyDeviceToHost,
@@ -562,32 +593,54 @@ void adjoint_t(const lgh_sv *psi_in, lgh_sv *const *bras, int nobs, const lgh_op
 
 extern "C" {
 
+static void adjoint_entry(const lgh_sv *psi, lgh_sv *const *bras, int32_t num_obs, const lgh_op *ops,
+                      int64_t num_ops, const int64_t *trainable, int64_t num_trainable, int32_t apply_operations,
+                      double *jac, const uint64_t *zmasks) {
+    LGH_REQUIRE(psi, "null state");
+    LGH_REQUIRE(num_trainable > 0, "No trainable parameters provided.");
+    LGH_REQUIRE(trainable && (num_obs == 0 || (bras && jac)) && (num_ops == 0 || ops), "null pointer");
+    LGH_REQUIRE(num_obs >= 0, "negative observable count");
+    for (int32_t j = 0; j < num_obs; ++j) {
+        LGH_REQUIRE(bras[j], "null observable state");
+        LGH_REQUIRE(bras[j]->n == psi->n && bras[j]->dtype == psi->dtype && bras[j]->device == psi->device,
+                    "observable state does not match psi");
+    }
+    for (int64_t i = 1; i < num_trainable; ++i)
+        LGH_REQUIRE(trainable[i] > trainable[i - 1], "trainable parameters must be sorted and unique");
+    for (int64_t i = 0; i < num_ops; ++i) {
+        if (ops[i].gate < 0) continue; // state preparation: skipped by the reverse sweep
+        LGH_REQUIRE(ops[i].num_params <= 1,
+                    "The operation is not supported using the adjoint differentiation method");
+        validate_op(ops[i], psi->n);
+    }
+    if (zmasks)
+        for (int32_t j = 0; j < num_obs; ++j)
+            LGH_REQUIRE((zmasks[j] >> psi->n) == 0, "Z-word mask beyond the state's qubits");
+    DeviceGuard g(psi->device);
+    if (psi->dtype == LGH_C128)
+        adjoint_t<double>(psi, bras, num_obs, ops, num_ops, trainable, num_trainable, apply_operations != 0, jac,
+                          zmasks);
+    else
+        adjoint_t<float>(psi, bras, num_obs, ops, num_ops, trainable, num_trainable, apply_operations != 0, jac,
+                         zmasks);
+}
+
 lgh_status lgh_adjoint_jacobian(const lgh_sv *psi, lgh_sv *const *bras, int32_t num_obs, const lgh_op *ops,
                                 int64_t num_ops, const int64_t *trainable, int64_t num_trainable,
                                 int32_t apply_operations, double *jac) {
     return guard([&] {
-        LGH_REQUIRE(psi, "null state");
-        LGH_REQUIRE(num_trainable > 0, "No trainable parameters provided.");
-        LGH_REQUIRE(trainable && (num_obs == 0 || (bras && jac)) && (num_ops == 0 || ops), "null pointer");
-        LGH_REQUIRE(num_obs >= 0, "negative observable count");
-        for (int32_t j = 0; j < num_obs; ++j) {
-            LGH_REQUIRE(bras[j], "null observable state");
-            LGH_REQUIRE(bras[j]->n == psi->n && bras[j]->dtype == psi->dtype && bras[j]->device == psi->device,
-                        "observable state does not match psi");
-        }
-        for (int64_t i = 1; i < num_trainable; ++i)
-            LGH_REQUIRE(trainable[i] > trainable[i - 1], "trainable parameters must be sorted and unique");
-        for (int64_t i = 0; i < num_ops; ++i) {
-            if (ops[i].gate < 0) continue; // state preparation: skipped by the reverse sweep
-            LGH_REQUIRE(ops[i].num_params <= 1,
-                        "The operation is not supported using the adjoint differentiation method");
-            validate_op(ops[i], psi->n);
-        }
-        DeviceGuard g(psi->device);
-        if (psi->dtype == LGH_C128)
-            adjoint_t<double>(psi, bras, num_obs, ops, num_ops, trainable, num_trainable, apply_operations != 0, jac);
-        else
-            adjoint_t<float>(psi, bras, num_obs, ops, num_ops, trainable, num_trainable, apply_operations != 0, jac);
+        Range rg("lgh_adjoint_jacobian");
+        adjoint_entry(psi, bras, num_obs, ops, num_ops, trainable, num_trainable, apply_operations, jac, nullptr);
+    });
+}
+
+lgh_status lgh_adjoint_jacobian_zwords(const lgh_sv *psi, lgh_sv *const *bras, const uint64_t *zmasks,
+                                       int32_t num_obs, const lgh_op *ops, int64_t num_ops, const int64_t *trainable,
+                                       int64_t num_trainable, int32_t apply_operations, double *jac) {
+    return guard([&] {
+        Range rg("lgh_adjoint_jacobian_zwords");
+        LGH_REQUIRE(num_obs == 0 || zmasks, "null pointer");
+        adjoint_entry(psi, bras, num_obs, ops, num_ops, trainable, num_trainable, apply_operations, jac, zmasks);
     });
 }
 

@@ -802,6 +802,7 @@ lgh_status lgh_batch_plan_create(lgh_batch_plan **out, lgh_tmpl *const *tmpls, i
 lgh_status lgh_batch_plan_run(lgh_batch_plan *plan, const double *params, const int64_t *param_offset,
                               double *expval, double *jac, const int64_t *jac_offset) {
     return guard([&] {
+        Range rg("lgh_batch_plan_run");
         LGH_REQUIRE(plan && expval, "null pointer");
         const BatchLayout &L = plan->L;
         LGH_REQUIRE(!jac || L.grad, "the plan was created without gradients");
@@ -1048,6 +1049,7 @@ lgh_status lgh_batch_run(lgh_tmpl *const *tmpls, int32_t num_circuits, lgh_dtype
                          const int64_t *param_offset, double *expval, double *jac, const int64_t *jac_offset,
                          void *stream) {
     return guard([&] {
+        Range rg("lgh_batch_run");
         check_batch_args(tmpls, num_circuits, dtype, params, param_offset, expval, jac, jac_offset);
         if (num_circuits == 0) return;
         if (dtype == LGH_C128)

@@ -418,11 +418,16 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
 // first pass of a sweep, which then needs no copy of it); lam: where it is
 // written.  Each thread reads and writes its own amplitudes only, so the two
 // may be the same array.
+// bra_zmask != 0 with bra_from_lam: the observable state is not read but
+// formed from lambda's loaded values as Z-word(bra_zmask) lambda (the sign
+// (-1)^popcount(index & bra_zmask) per amplitude), then written like any
+// pass's output: the first reverse pass of a sweep with a Z-word observable
+// needs no separate pass that forms it.
 template <class R, int RB>
 __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                const FOp *__restrict__ ops, const FGen *__restrict__ gens,
-                                               double *__restrict__ partials) {
+                                               double *__restrict__ partials, uint64_t bra_zmask, int bra_from_lam) {
     __shared__ double acc[kMaxSlots][kBlock];
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const CFOp *cops = (const CFOp *)ops;
@@ -447,10 +452,19 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
         cx<R> a[NR], b[NR];
         cx<R> *pa = lam + base, *pb = bra + base; // the tile bits are disjoint
         const cx<R> *qa = lam_in + base;
+        if (bra_from_lam) {
 #pragma unroll
-        for (int k = 0; k < NR; ++k) {
-            a[k] = ldnt(qa + roff[k]);
-            b[k] = ldnt(pb + roff[k]);
+            for (int k = 0; k < NR; ++k) {
+                a[k] = ldnt(qa + roff[k]);
+                const uint32_t m = (__popcll((base | roff[k]) & bra_zmask) & 1) ? kSignBit : 0u;
+                b[k] = cx<R>{flip(a[k].re, m), flip(a[k].im, m)};
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < NR; ++k) {
+                a[k] = ldnt(qa + roff[k]);
+                b[k] = ldnt(pb + roff[k]);
+            }
         }
         FOp cur = load_op(cops);
         wait_op(cur);
@@ -513,15 +527,17 @@ template <class R>
 __global__ __launch_bounds__(kBlock, 3) void k_fused_adj3(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                        uint64_t ntiles, uint64_t chunk,
                                                        const PassDesc *__restrict__ pd, const FOp *__restrict__ ops,
-                                                       const FGen *__restrict__ gens, double *__restrict__ partials) {
-    fused_adj_body<R, 3>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials);
+                                                       const FGen *__restrict__ gens, double *__restrict__ partials,
+                                                       uint64_t bra_zmask, int bra_from_lam) {
+    fused_adj_body<R, 3>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
 }
 template <class R>
 __global__ __launch_bounds__(kBlock) void k_fused_adj4(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                     uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                     const FOp *__restrict__ ops, const FGen *__restrict__ gens,
-                                                    double *__restrict__ partials) {
-    fused_adj_body<R, 4>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials);
+                                                    double *__restrict__ partials, uint64_t bra_zmask,
+                                                    int bra_from_lam) {
+    fused_adj_body<R, 4>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
 }
 
 // slot sums -> jac[col[s]] * factor[s]
@@ -1072,7 +1088,7 @@ template <class R>
 static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std::vector<KOp> &udag,
                         const std::vector<KGen> &gens, const std::vector<int> &slot_col,
                         const std::vector<double> &factor, double *jac_row_dev, double *partials,
-                        StandaloneStep standalone) {
+                        StandaloneStep standalone, const BraFromZ *bra_z) {
     // `udag` lists the ops in forward order with U^dagger matrices; the passes
     // are scheduled on that order and executed backwards.
     StreamCtx *ctx = lam->ctx;
@@ -1083,9 +1099,15 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
     const Geometry geo = geometry(n, lam->device, rb, cfg.waves_per_cu, g_waves_adj.load());
     // lambda still to be read from lam_src (until the first pass has run)
     const void *src = lam_src;
+    // the observable state still to be formed from lambda (first pass only)
+    bool bra_pending = bra_z != nullptr;
     for (auto it = passes->rbegin(); it != passes->rend(); ++it) {
         const Pass &p = *it;
         if (p.standalone) {
+            if (bra_pending) {
+                bra_z->materialize(); // before lambda's first change
+                bra_pending = false;
+            }
             if (src) {
                 LGH_HIP(hipMemcpyAsync(lam->data, src, lam->bytes, hipMemcpyDeviceToDevice, ctx->stream));
                 src = nullptr;
@@ -1119,10 +1141,12 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
             auto kern = rb == 4 ? k_fused_adj4<R> : k_fused_adj3<R>;
             const cx<R> *lam_in = static_cast<const cx<R> *>(src ? src : lam->data);
             src = nullptr;
+            const uint64_t zmask = bra_pending ? bra_z->zmask : 0ull;
             hipLaunchKernelGGL(kern, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream, lam_in,
                                static_cast<cx<R> *>(lam->data), static_cast<cx<R> *>(bra->data), geo.ntiles,
                                geo.chunk, reinterpret_cast<const PassDesc *>(d), reinterpret_cast<const FOp *>(d + pb),
-                               reinterpret_cast<const FGen *>(d + pb + ob), partials);
+                               reinterpret_cast<const FGen *>(d + pb + ob), partials, zmask, int(bra_pending));
+            bra_pending = false;
             LGH_LAUNCH_CHECK();
         }
         if (!cols.empty()) {
@@ -1137,11 +1161,13 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
 void fused_adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std::vector<KOp> &udag,
                        const std::vector<KGen> &gens, const std::vector<int> &slot_col,
                        const std::vector<double> &factor, double *jac_row_dev, double *partials,
-                       StandaloneStep standalone) {
+                       StandaloneStep standalone, const BraFromZ *bra_z) {
     if (lam->dtype == LGH_C128)
-        adjoint_one<double>(lam, lam_src, bra, udag, gens, slot_col, factor, jac_row_dev, partials, standalone);
+        adjoint_one<double>(lam, lam_src, bra, udag, gens, slot_col, factor, jac_row_dev, partials, standalone,
+                            bra_z);
     else
-        adjoint_one<float>(lam, lam_src, bra, udag, gens, slot_col, factor, jac_row_dev, partials, standalone);
+        adjoint_one<float>(lam, lam_src, bra, udag, gens, slot_col, factor, jac_row_dev, partials, standalone,
+                           bra_z);
 }
 
 size_t fused_partials_bytes(int n, int device) {

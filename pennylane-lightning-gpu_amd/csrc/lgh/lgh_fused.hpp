@@ -58,6 +58,14 @@ void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops);
 // state; with a generator, stores factor * Im<bra|G lambda> into jac_row[col].
 using StandaloneStep = std::function<void(const KOp &udag, const KGen *gen, int64_t col, double factor)>;
 
+// An observable state that the sweep forms itself: bra = Z-word(zmask)
+// lambda, inside the first reverse pass when that pass is fused; otherwise
+// `materialize` forms it before lambda first changes.
+struct BraFromZ {
+    uint64_t zmask;
+    std::function<void()> materialize;
+};
+
 // Reverse sweep of one observable state through fused passes.  udag: the
 // ops in forward order compiled as U^dagger; slot_col[i]: Jacobian column of
 // a trainable gate or -1; factor[i] = -2 s (inverse ? -1 : 1).  lam_src: if
@@ -66,7 +74,7 @@ using StandaloneStep = std::function<void(const KOp &udag, const KGen *gen, int6
 void fused_adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std::vector<KOp> &udag,
                        const std::vector<KGen> &gens, const std::vector<int> &slot_col,
                        const std::vector<double> &factor, double *jac_row_dev, double *partials,
-                       StandaloneStep standalone);
+                       StandaloneStep standalone, const BraFromZ *bra_z = nullptr);
 size_t fused_partials_bytes(int n, int device);
 
 } // namespace lgh

@@ -540,6 +540,7 @@ extern "C" {
 
 lgh_status lgh_apply(lgh_sv *sv, const lgh_op *ops, int64_t num_ops) {
     return guard([&] {
+        Range rg("lgh_apply");
         LGH_REQUIRE(sv, "null state");
         LGH_REQUIRE(num_ops == 0 || ops, "null op list");
         // validate everything before launching anything

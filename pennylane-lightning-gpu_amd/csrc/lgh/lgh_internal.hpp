@@ -271,6 +271,23 @@ struct DeviceGuard {
 
 int sm_count(int device);
 
+// roctx ranges around the library's entry points (lgh_apply, the adjoint
+// sweep, the measurements, batched runs), for timelines recorded with
+// `rocprofv3 --marker-trace`.  Off unless the environment sets LGH_ROCTX=1,
+// so that an untraced run pays one branch per call.
+bool roctx_enabled();
+void roctx_push(const char *name);
+void roctx_pop();
+struct Range {
+    bool on;
+    explicit Range(const char *name) : on(roctx_enabled()) {
+        if (on) roctx_push(name);
+    }
+    ~Range() {
+        if (on) roctx_pop();
+    }
+};
+
 // Optional launch timing (lgh_profile_enable): HIP events recorded on the
 // launch stream around each launch of the hot kernels, summed per kind by
 // lgh_profile_read.

@@ -1220,6 +1220,7 @@ lgh_status lgh_scale(lgh_sv *x, double are, double aim) {
 lgh_status lgh_expval_matrix(const lgh_sv *sv, const int32_t *wires, int32_t num_wires, const double *matrix,
                              double *re, double *im) {
     return guard([&] {
+        Range rg("lgh_expval_matrix");
         LGH_REQUIRE(sv && wires && matrix && re && im, "null pointer");
         LGH_REQUIRE(num_wires >= 1 && num_wires <= kMaxLdsK && num_wires <= sv->n,
                     "observable must act on 1..13 wires of the state");
@@ -1235,6 +1236,7 @@ lgh_status lgh_expval_matrix(const lgh_sv *sv, const int32_t *wires, int32_t num
 lgh_status lgh_expval_pauli_words(const lgh_sv *sv, int64_t num_words, const char *paulis, const int32_t *wires,
                                   const int64_t *offsets, const double *coeffs, double *per_word, double *result) {
     return guard([&] {
+        Range rg("lgh_expval_pauli_words");
         LGH_REQUIRE(sv && result, "null pointer");
         LGH_REQUIRE(num_words >= 0, "negative word count");
         LGH_REQUIRE(num_words == 0 || (paulis && offsets && coeffs), "null pointer");
@@ -1354,6 +1356,7 @@ lgh_status lgh_csr_destroy(lgh_csr *csr) {
 
 lgh_status lgh_expval_csr(const lgh_sv *sv, const lgh_csr *csr, double *re, double *im) {
     return guard([&] {
+        Range rg("lgh_expval_csr");
         LGH_REQUIRE(sv && csr && re && im, "null pointer");
         LGH_REQUIRE(csr->rows == int64_t(sv->length()), "CSR dimension does not match the state");
         LGH_REQUIRE(csr->dtype == sv->dtype && csr->device == sv->device, "CSR precision/device mismatch");
@@ -1377,6 +1380,7 @@ lgh_status lgh_apply_csr(lgh_sv *out, const lgh_csr *csr, const lgh_sv *in) {
 
 lgh_status lgh_probs(const lgh_sv *sv, const int32_t *wires, int32_t num_wires, double *out) {
     return guard([&] {
+        Range rg("lgh_probs");
         LGH_REQUIRE(sv && out && (num_wires == 0 || wires), "null pointer");
         LGH_REQUIRE(num_wires >= 0 && num_wires <= sv->n && num_wires <= 30, "bad wire count");
         for (int i = 0; i < num_wires; ++i) {
@@ -1390,6 +1394,7 @@ lgh_status lgh_probs(const lgh_sv *sv, const int32_t *wires, int32_t num_wires, 
 
 lgh_status lgh_sample(const lgh_sv *sv, uint64_t shots, uint64_t seed, uint64_t *indices) {
     return guard([&] {
+        Range rg("lgh_sample");
         LGH_REQUIRE(sv && (shots == 0 || indices), "null pointer");
         LGH_REQUIRE(shots < (1ull << 32), "too many shots in one call");
         if (shots == 0) return;

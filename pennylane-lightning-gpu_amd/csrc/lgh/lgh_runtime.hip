@@ -11,6 +11,10 @@
 #include <map>
 #include <unordered_map>
 
+#include <cstdlib>
+
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "lgh_internal.hpp"
 
 namespace lgh {
@@ -26,6 +30,16 @@ DeviceGuard::~DeviceGuard() {
     int cur = -1;
     if (hipGetDevice(&cur) == hipSuccess && cur != prev && prev >= 0) (void)hipSetDevice(prev);
 }
+
+bool roctx_enabled() {
+    static const bool on = [] {
+        const char *e = std::getenv("LGH_ROCTX");
+        return e && e[0] == '1';
+    }();
+    return on;
+}
+void roctx_push(const char *name) { roctxRangePush(name); }
+void roctx_pop() { roctxRangePop(); }
 
 int sm_count(int device) {
     static std::mutex mu;

@@ -114,6 +114,8 @@ def lib():
         L.lgh_sample_bits.argtypes = [vp, C.c_uint64, C.c_uint64, vp]
         L.lgh_adjoint_jacobian.argtypes = [vp, C.POINTER(vp), C.c_int32, C.POINTER(Op), C.c_int64, vp,
                                            C.c_int64, C.c_int32, vp]
+        L.lgh_adjoint_jacobian_zwords.argtypes = [vp, C.POINTER(vp), vp, C.c_int32, C.POINTER(Op), C.c_int64, vp,
+                                                  C.c_int64, C.c_int32, vp]
         L.lgh_device_count.argtypes = [C.POINTER(C.c_int)]
         L.lgh_device_arch.argtypes = [C.c_int, C.c_char_p, C.c_size_t]
         L.lgh_device_supported.argtypes = [C.c_int, C.POINTER(C.c_int)]
@@ -477,6 +479,21 @@ def adjoint_jacobian(psi: State, bras, ops, trainable, apply_operations=False) -
     jac = np.zeros(max(1, len(bras) * len(trainable)))
     check(lib().lgh_adjoint_jacobian(psi.handle, hs, len(bras), arr, len(ops), tp.ctypes.data, len(trainable),
                                      int(apply_operations), jac.ctypes.data))
+    return jac[: len(bras) * len(trainable)].reshape(len(bras), len(trainable))
+
+
+def adjoint_jacobian_zwords(psi: State, zwires, ops, trainable, apply_operations=False) -> np.ndarray:
+    """Jacobian [obs][trainable] for observables that are Z words: zwires[j]
+    lists the wires of observable j's Z factors.  The observable states are
+    scratch states formed by the sweep itself (lgh_adjoint_jacobian_zwords)."""
+    arr, keep = make_ops(ops)
+    bras = [State(psi.n, psi.dtype) for _ in zwires]
+    hs = (C.c_void_p * max(1, len(bras)))(*[b.handle.value for b in bras])
+    zm = np.ascontiguousarray([sum(1 << (psi.n - 1 - int(w)) for w in ws) for ws in zwires] or [0], dtype=np.uint64)
+    tp = np.ascontiguousarray(list(trainable) or [0], dtype=np.int64)
+    jac = np.zeros(max(1, len(bras) * len(trainable)))
+    check(lib().lgh_adjoint_jacobian_zwords(psi.handle, hs, zm.ctypes.data, len(bras), arr, len(ops), tp.ctypes.data,
+                                            len(trainable), int(apply_operations), jac.ctypes.data))
     return jac[: len(bras) * len(trainable)].reshape(len(bras), len(trainable))
 
 

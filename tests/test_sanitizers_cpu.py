@@ -66,7 +66,7 @@ def build_driver():
     exe = os.path.join(OUT, "host_driver")
     if jobs or not os.path.exists(exe):
         _run([CLANGXX, "-fsanitize=address,undefined", *objs, drv_obj, orc_obj, "-L/opt/rocm/lib", "-lamdhip64",
-              "-lm", "-Wl,-rpath,/opt/rocm/lib", "-o", exe])
+              "-lrocprofiler-sdk-roctx", "-lm", "-Wl,-rpath,/opt/rocm/lib", "-o", exe])
     return exe
 
 
