@@ -82,8 +82,8 @@ def parse():
     ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
-    ap.add_argument("--batch-serial-sample", type=int, default=256,
-                    help="circuits timed one after another for the per-circuit comparison")
+    ap.add_argument("--batch-serial-sample", type=int, default=0,
+                    help="circuits timed one after another for the per-circuit comparison (0: all of them)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run device-to-device copies of 1 GiB (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -171,7 +171,7 @@ def bench_batched(N, args, dtype):
         log(f"[bench] batched {name} step {i}: {steps[-1] * 1e3:.1f} ms")
     kn, kms = N.profile_read("batch")
     N.profile_enable(False)
-    k = min(args.batch_serial_sample, len(ts))
+    k = len(ts) if args.batch_serial_sample <= 0 else min(args.batch_serial_sample, len(ts))
     t0 = time.perf_counter()
     ev_s, jac_s = N.batch_run(ts[:k], thetas[:k], serial=True, dtype=dtype)
     t_serial = time.perf_counter() - t0
