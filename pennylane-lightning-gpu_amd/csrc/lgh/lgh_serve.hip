@@ -83,6 +83,12 @@ bool template_gate(int g) {
     }
 }
 
+// Batches hold circuits of one width class: a persistent launch runs each
+// circuit on one workgroup and lasts as long as its largest circuit, so a
+// batch of 10-qubit circuits should not wait for 20-qubit ones, which get
+// batches of their own (on another worker, at the same time).
+int width_class(int n) { return n <= 12 ? 0 : n <= 16 ? 1 : 2; }
+
 bool parametric(int g) {
     switch (g) {
     case LGH_GATE_RX: case LGH_GATE_RY: case LGH_GATE_RZ: case LGH_GATE_PHASESHIFT: case LGH_GATE_CRX:
@@ -212,8 +218,9 @@ struct lgh_server {
     }
 
     // Under mu: the next piece of work that fits the memory budget, or
-    // nothing.  Waiting small circuits go first, all of them (up to
-    // max_batch) into one batch, while fewer than max_batches batches run;
+    // nothing.  Waiting small circuits go first: those of the oldest one's
+    // width class (up to max_batch) into one batch, while fewer than
+    // max_batches batches run;
     // otherwise the oldest large circuit that fits, while fewer than
     // max_singles run (with several workers one always stays free for the
     // small circuits, whose latency would otherwise be set by the largest
@@ -221,17 +228,22 @@ struct lgh_server {
     // memory.
     bool pick(std::vector<Req *> &batch, Req *&single, uint64_t &bytes) {
         if (!small.empty() && running_batches < max_batches) {
+            // the oldest waiting circuit's width class, its peers in arrival order
+            const int cls = width_class(small.front()->n);
             uint64_t b = 0;
-            size_t k = 0;
-            while (k < small.size() && k < size_t(max_batch) &&
-                   (k == 0 ? (reserved + small[0]->bytes <= budget || reserved == 0)
-                           : reserved + b + small[k]->bytes <= budget)) {
-                b += small[k]->bytes;
-                ++k;
+            std::deque<Req *> rest;
+            for (Req *r : small) {
+                const bool fits = batch.empty() ? (reserved + r->bytes <= budget || reserved == 0)
+                                                : reserved + b + r->bytes <= budget;
+                if (width_class(r->n) == cls && batch.size() < size_t(max_batch) && fits) {
+                    batch.push_back(r);
+                    b += r->bytes;
+                } else {
+                    rest.push_back(r);
+                }
             }
-            if (k > 0) {
-                batch.assign(small.begin(), small.begin() + long(k));
-                small.erase(small.begin(), small.begin() + long(k));
+            if (!batch.empty()) {
+                small.swap(rest);
                 bytes = b;
                 ++running_batches;
                 return true;
