@@ -661,11 +661,12 @@ def main():
     N.profile_enable(False)
 
     # ---------------------------------------------------------------- dominant kernel: fused adjoint pass
-    # One launch reads and writes lambda and the observable state once:
-    # algorithmic bytes = 4 * state bytes per launch, whatever the number of
-    # gates fused into it.
+    # A launch reads and writes lambda and the observable state once: 4 state
+    # bytes per launch, whatever the number of gates fused into it.  The
+    # observable Z0 Z27 is a Z word, which the first reverse pass of each step
+    # forms from lambda instead of reading it: 3 state bytes for that launch.
     adj_avg = adj_ms / max(1, adj_n)
-    fused_bytes = 4 * state_bytes
+    fused_bytes = (4 * adj_n - args.steps) * state_bytes / max(1, adj_n)  # per launch, averaged
     fused_gbs = fused_bytes / (adj_avg * 1e-3) / 1e9
     fwd_avg = fwd_ms / max(1, fwd_n)
     # HBM bytes per launch from the committed PMC summary, used only if it
@@ -681,7 +682,7 @@ def main():
             traffic_note = "PMC summary predates the current kernel sources: not used"
     kernel_stats = {
         "k_fused_adj": {"launches_per_step": adj_n / args.steps, "avg_ms": round(adj_avg, 4),
-                        "ms_per_step": round(adj_ms / args.steps, 2), "bytes_per_launch": fused_bytes,
+                        "ms_per_step": round(adj_ms / args.steps, 2), "bytes_per_launch": int(fused_bytes),
                         "GBps": round(fused_gbs, 1)},
         "k_fused_pass": {"launches_per_step": fwd_n / args.steps, "avg_ms": round(fwd_avg, 4),
                          "ms_per_step": round(fwd_ms / args.steps, 2), "bytes_per_launch": 2 * state_bytes,
@@ -752,7 +753,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_fused_adj<double> (fused adjoint pass: lambda + 1 observable state)",
                      "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fused_gbs / HBM_PEAK_GBS, 3), "traffic": traffic, "traffic_note": traffic_note,
-                     "bytes_per_launch": fused_bytes, "ms_per_launch": round(adj_avg, 4)},
+                     "bytes_per_launch": int(fused_bytes), "ms_per_launch": round(adj_avg, 4)},
         "cpu_baseline": cpu,
         "autotune": tune,
         "kernels": kernel_stats,
