@@ -80,6 +80,10 @@ def parse():
                     help="offered rates (requests/s) of a sustainable-rate sweep")
     ap.add_argument("--serve-sweep-arrivals", type=int, default=600)
     ap.add_argument("--serve-only", action="store_true", help="run only the serving workload")
+    ap.add_argument("--published-wires", type=int, nargs="*", default=[21, 23, 25, 26, 27],
+                    help="wire counts of the reference's published StronglyEntanglingLayers workload")
+    ap.add_argument("--no-published", action="store_true")
+    ap.add_argument("--published-only", action="store_true", help="run only the published workload")
     ap.add_argument("--batch-circuits", type=int, default=4096)
     ap.add_argument("--batch-steps", type=int, default=3)
     ap.add_argument("--batch-serial-sample", type=int, default=0,
@@ -304,6 +308,83 @@ def bench_measurements(N, args):
                             "circuits_per_s": round(256 / (ms * 1e-3), 1)}
     t.close()
     return out
+
+
+# The reference's published adjoint benchmark: qml.jacobian wall time of
+# StronglyEntanglingLayers with <Z_w> on every wire, complex128, one A100,
+# read off the log-scale bar chart doc/_static/lightning_gpu_initial_bm.png
+# (.github/CHANGELOG.md:603-640) to about +-5 %.  Keys: (wires, layers).
+PUBLISHED_A100_S = {(21, 1): 2.7, (21, 2): 4.5, (23, 1): 5.6, (23, 2): 8.0,
+                    (25, 1): 16.0, (25, 2): 19.0, (27, 1): 57.0, (27, 2): 72.0}
+
+
+def bench_published(N, ops_mod, args):
+    """The reference's published workload shape on this build, through the
+    pybind11 API as the device drives it: reset, the StronglyEntanglingLayers
+    forward, <Z_w> for every wire, and the n x 3nL adjoint Jacobian of all n
+    observables.  One warm-up, median of 3.  The fused adjoint passes of the
+    timed runs are timed with HIP events: each observable's sweep forms its
+    observable state inside its first pass (3 state bytes) and moves 4 state
+    bytes in every other pass."""
+    import numpy as np
+
+    from pennylane_lightning_gpu.workloads import strongly_entangling
+
+    z = np.diag([1.0, -1.0]).astype(np.complex128).ravel()
+    rows = []
+    for n in args.published_wires:
+        for layers in (1, 2):
+            ops = strongly_entangling(n, layers, seed=args.seed)
+            n_par = 3 * n * layers
+            S = 16 << n
+            sv = ops_mod.LightningGPU_C128(n)
+            adj = ops_mod.AdjointJacobianGPU_C128()
+            obs = [ops_mod.NamedObsGPU_C128("PauliZ", [w]) for w in range(n)]
+            opsl = adj.create_ops_list([o[0] for o in ops], [np.array(o[2], dtype=np.float64) for o in ops],
+                                       [o[1] for o in ops], [o[3] for o in ops],
+                                       [np.array([], complex) for _ in ops])
+            names, wires, invs, params = ([o[0] for o in ops], [o[1] for o in ops], [o[3] for o in ops],
+                                          [o[2] for o in ops])
+
+            def run():
+                sv.resetGPU()
+                sv.apply(names, wires, invs, params)
+                ev = [sv.ExpectationValue("PauliZ", [w], [], z) for w in range(n)]
+                return ev, np.asarray(adj.adjoint_jacobian(sv, obs, opsl, list(range(n_par))))
+
+            run()
+            N.check(N.lib().lgh_device_synchronize(0))
+            N.profile_enable(True)
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                ev, jac = run()
+                N.check(N.lib().lgh_device_synchronize(0))
+                ts.append(time.perf_counter() - t0)
+            a_n, a_ms = N.profile_read("fused_adjoint")
+            f_n, f_ms = N.profile_read("fused_pass")
+            N.profile_enable(False)
+            sec = sorted(ts)[1]
+            a_bytes = (4 * a_n - 3 * n) * S / max(1, a_n)  # per launch, averaged over 3 runs
+            pub = PUBLISHED_A100_S.get((n, layers))
+            rows.append({
+                "wires": n, "layers": layers, "params": n_par, "observables": n, "s": round(sec, 4),
+                "published_a100_s": pub, "ratio_vs_published": round(pub / sec, 1) if pub else None,
+                "adjoint_launches": a_n // 3, "adjoint_ms_per_launch": round(a_ms / max(1, a_n), 4),
+                "adjoint_bytes_per_launch": int(a_bytes),
+                "adjoint_GBps": round(a_bytes / (a_ms / max(1, a_n) * 1e-3) / 1e9, 1) if a_n else None,
+                "adjoint_ms": round(a_ms / 3, 2), "forward_launches": f_n // 3, "forward_ms": round(f_ms / 3, 2),
+                "jacobian_checksum": float(jac.sum()), "expval_checksum": float(np.sum(ev)),
+            })
+            log(f"[bench] published shape {n} wires x {layers} layers: {sec:.3f} s "
+                f"(A100 published {pub} s), {a_n // 3} adjoint passes")
+            del sv, adj, obs, opsl
+            N.check(N.lib().lgh_pool_trim(0))
+    return {"workload": "StronglyEntanglingLayers (Rot expanded to RZ RY RZ, CNOT ranges (l mod (n-1)) + 1), "
+                        "<Z_w> on every wire, complex128, forward + expectation values + adjoint Jacobian",
+            "published_note": "A100 figures are qml.jacobian wall times including PennyLane overhead, read off "
+                              "the reference's bar chart (about +-5 %); this build's times exclude PennyLane",
+            "rows": rows}
 
 
 def serving_requests(args, arrivals, rate, seed, max_qubits=None):
@@ -611,6 +692,9 @@ def main():
     if args.serve_only:
         print(json.dumps({"serving": bench_serving(N, args)}))
         return
+    if args.published_only:
+        print(json.dumps({"published_workload": bench_published(N, ops_mod, args)}))
+        return
     n, layers = args.qubits, args.layers
     ops = ry_rz_cz_brick(n, layers, seed=args.seed)
     n_par = sum(1 for o in ops if o[2])
@@ -727,6 +811,12 @@ def main():
                  "median_GBps": float(np.median(gbs)), "max_GBps": max(gbs),
                  "min_frac_of_peak": round(min(gbs) / HBM_PEAK_GBS, 3), "per_target": rows}
         del s
+        N.check(N.lib().lgh_pool_trim(0))
+
+    # ---------------------------------------------------------------- the reference's published workload shape
+    published = None
+    if not args.no_published:
+        published = bench_published(N, ops_mod, args)
 
     # ---------------------------------------------------------------- CPU oracle baseline (measured components)
     cpu = None
@@ -761,6 +851,7 @@ def main():
         "serving": serving,
         "measurements": measurements,
         "gate_sweep": sweep,
+        "published_workload": published,
         "jacobian_checksum": float(np.sum(jac)),
     }
     print(json.dumps(rec))

@@ -45,6 +45,26 @@ def ry_rz_cz_brick(n: int, layers: int, seed: int = 1337):
     return ops
 
 
+def strongly_entangling(n: int, layers: int, seed: int = 1337):
+    """PennyLane's StronglyEntanglingLayers as the device serializes it: per
+    layer l, Rot(phi, theta, omega) on every wire, expanded to RZ(phi), RY(theta),
+    RZ(omega) (reference _serialize.py:311-312), then CNOT(w, (w + r) mod n) for
+    every wire with range r = (l mod (n-1)) + 1.  3 n L parameters; the
+    reference's published adjoint benchmark differentiates <Z_w> on every wire
+    of this circuit (.github/CHANGELOG.md:603-640)."""
+    rng = np.random.default_rng(seed)
+    ops = []
+    for layer in range(layers):
+        for w in range(n):
+            for g in ("RZ", "RY", "RZ"):
+                ops.append((g, [w], [float(rng.uniform(0, TWO_PI))], False))
+        if n > 1:
+            r = layer % (n - 1) + 1
+            for w in range(n):
+                ops.append(("CNOT", [w, (w + r) % n], [], False))
+    return ops
+
+
 def ladder(n: int, layers: int, seed: int = 1337):
     """Per layer: RX, RY, RZ on every wire, then CNOT(w, w+1) for w < n-1."""
     rng = np.random.default_rng(seed)

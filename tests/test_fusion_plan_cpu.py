@@ -91,3 +91,20 @@ def test_pauli_plan_groups_words_by_tile_bits():
     assert wide == {"tile_passes": 1, "tile_launches": 1, "pair_launches": 1}
     small = N.pauli_plan(4, ["XXXX", "YZ", "Z"], [[0, 1, 2, 3], [0, 3], [2]])
     assert small == {"tile_passes": 1, "tile_launches": 1, "pair_launches": 0}
+
+
+@pytest.mark.parametrize("n,layers", [(21, 1), (27, 2)])
+def test_strongly_entangling_plan(n, layers):
+    """StronglyEntanglingLayers (Rot expanded, CNOT ranges 1 and 2 with
+    wrap-around) fuses completely: no standalone pass, every RZ and no CNOT in
+    a diagonal run, few passes each way."""
+    from pennylane_lightning_gpu.workloads import strongly_entangling
+
+    ops = strongly_entangling(n, layers, seed=1337)
+    assert len(ops) == 4 * n * layers and sum(1 for o in ops if o[2]) == 3 * n * layers
+    ranges = {(o[1][1] - o[1][0]) % n for o in ops if o[0] == "CNOT"}
+    assert ranges == set(range(1, layers + 1))
+    fwd, adj = plan(n, ops), plan(n, ops, adjoint=True)
+    assert fwd["standalone"] == 0 and adj["standalone"] == 0
+    assert fwd["passes"] <= 3 * layers and adj["passes"] <= 6 * layers
+    assert adj["run_members"] == 2 * n * layers

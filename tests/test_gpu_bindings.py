@@ -274,3 +274,23 @@ def test_adjoint_hermitian_and_sparse_vs_oracle(prefix):
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-10 * max(1.0, np.max(np.abs(ref))))
     else:
         np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * max(1.0, np.max(np.abs(ref))))
+
+
+@pytest.mark.parametrize("n,prefix", [(12, "C128"), (12, "C64"), (16, "C128")])
+def test_adjoint_strongly_entangling_all_wires_vs_oracle(n, prefix):
+    """The reference's published adjoint workload shape (.github/CHANGELOG.md:
+    603-640): StronglyEntanglingLayers, Rot expanded to RZ RY RZ as the device
+    serializes it (_serialize.py:311-312), <Z_w> on every wire; the full
+    n x 3nL Jacobian against the oracle's adjoint."""
+    from pennylane_lightning_gpu.workloads import strongly_entangling
+
+    ops = strongly_entangling(n, 2, seed=77)
+    psi = po.apply_ops(n, po.zero_state(n), ops)
+    npar = 3 * n * 2
+    bras = [po.apply_ops(n, psi, [("PauliZ", [w], [], False)]) for w in range(n)]
+    ref = po.adjoint_jacobian(n, psi, bras, ops, list(range(npar)))
+    got = run_adjoint(n, ops, [obs(prefix, "NamedObs", "PauliZ", [w]) for w in range(n)], list(range(npar)),
+                      prefix=prefix)
+    assert got.shape == (n, npar)
+    tol = 1e-10 if prefix == "C128" else 1e-5
+    np.testing.assert_allclose(got, ref, rtol=0, atol=tol * max(1.0, np.max(np.abs(ref))))
