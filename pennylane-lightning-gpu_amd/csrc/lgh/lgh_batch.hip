@@ -921,7 +921,7 @@ lgh_status lgh_tmpl_create(lgh_tmpl **out, int32_t num_qubits, const lgh_op *ops
         };
         int nf = 0;
         {
-            const std::vector<Pass> passes = schedule_passes(fk, nq, {}, RB, kFixedLanes);
+            const std::vector<Pass> passes = schedule_passes(fk, nq, {}, RB, kFixedLanes, kMaxSlots);
             for (const Pass &p : passes) {
                 LGH_REQUIRE(!p.standalone, "gate does not fit a fused pass");
                 Built b = build_pass(p, fk, nq, false, nullptr, nullptr, RB, false);
@@ -955,7 +955,7 @@ lgh_status lgh_tmpl_create(lgh_tmpl **out, int32_t num_qubits, const lgh_op *ops
                     slot_of.push_back(-1);
                 }
             }
-            const std::vector<Pass> passes = schedule_passes(uk, nq, slot_of, RB, kFixedLanes);
+            const std::vector<Pass> passes = schedule_passes(uk, nq, slot_of, RB, kFixedLanes, kMaxSlots);
             for (auto it = passes.rbegin(); it != passes.rend(); ++it) {
                 const Pass &p = *it;
                 LGH_REQUIRE(!p.standalone, "gate does not fit a fused pass");
@@ -996,7 +996,7 @@ lgh_status lgh_tmpl_create(lgh_tmpl **out, int32_t num_qubits, const lgh_op *ops
             }
             Pass p;
             if (!ok.empty()) {
-                const std::vector<Pass> passes = schedule_passes(ok, nq, {}, RB);
+                const std::vector<Pass> passes = schedule_passes(ok, nq, {}, RB, 6, kMaxSlots);
                 LGH_REQUIRE(passes.size() == 1 && !passes[0].standalone,
                             "observable needs more than one pass (at most 4 X/Y factors on wires 0..n-7)");
                 p = passes[0];

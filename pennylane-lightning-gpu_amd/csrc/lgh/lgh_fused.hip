@@ -133,7 +133,8 @@ bool fusable(const KOp &k, int n, int rb) {
 
 // Greedy list scheduling of `ops` (already compiled) into passes.
 std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot,
-                                  int rb, int fixed_lanes) {
+                                  int rb, int fixed_lanes, int max_slots) {
+    LGH_REQUIRE(max_slots >= 1 && max_slots <= 254, "fused pass: bad number of slots");
     LGH_REQUIRE(fixed_lanes >= 1 && fixed_lanes <= kLaneBits, "fused pass: bad number of fixed lane bits");
     // index bits the scheduler chooses per pass: the free lane positions and the registers
     const int nfree = kLaneBits - fixed_lanes + rb;
@@ -204,7 +205,7 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
             progress = false;
             for (size_t r = 0; r < s.ready.size();) {
                 const int i = s.ready[r];
-                if (!fus[i] || (use[i].nondiag & ~lane_mask & ~reg) || (train[i] && s.slots >= kMaxSlots)) {
+                if (!fus[i] || (use[i].nondiag & ~lane_mask & ~reg) || (train[i] && s.slots >= max_slots)) {
                     ++r;
                     continue;
                 }
@@ -223,7 +224,7 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
     auto blocked_targets = [&](const State &s, uint64_t reg) {
         uint64_t t = 0;
         for (int i : s.ready)
-            if (fus[i] && dense[i] && !(train[i] && s.slots >= kMaxSlots)) t |= use[i].nondiag & ~lane_mask & ~reg;
+            if (fus[i] && dense[i] && !(train[i] && s.slots >= max_slots)) t |= use[i].nondiag & ~lane_mask & ~reg;
         return t;
     };
     while (remaining > 0) {
@@ -329,11 +330,12 @@ constexpr size_t kSchedCacheMax = 512;
 
 std::shared_ptr<const std::vector<Pass>> schedule_passes_cached(const std::vector<KOp> &ops, int n,
                                                                 const std::vector<int> &trainable_slot, int rb,
-                                                                int fixed_lanes) {
+                                                                int fixed_lanes, int max_slots) {
     std::string key;
-    key.reserve(16 + ops.size() * 24);
+    key.reserve(24 + ops.size() * 24);
     auto put = [&](uint64_t v) { key.append(reinterpret_cast<const char *>(&v), sizeof(v)); };
     put(uint64_t(n) | uint64_t(rb) << 8 | uint64_t(fixed_lanes) << 16 | uint64_t(ops.size()) << 24);
+    put(uint64_t(max_slots));
     for (size_t i = 0; i < ops.size(); ++i) {
         const KOp &k = ops[i];
         const bool train = !trainable_slot.empty() && trainable_slot[i] >= 0;
@@ -349,7 +351,8 @@ std::shared_ptr<const std::vector<Pass>> schedule_passes_cached(const std::vecto
         auto it = c.map.find(key);
         if (it != c.map.end()) return it->second;
     }
-    auto sched = std::make_shared<const std::vector<Pass>>(schedule_passes(ops, n, trainable_slot, rb, fixed_lanes));
+    auto sched =
+        std::make_shared<const std::vector<Pass>>(schedule_passes(ops, n, trainable_slot, rb, fixed_lanes, max_slots));
     std::lock_guard<std::mutex> lk(c.mu);
     if (c.map.size() >= kSchedCacheMax) c.map.clear();
     c.map.emplace(std::move(key), sched);
@@ -428,12 +431,17 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
                                                uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                double *__restrict__ partials, uint64_t bra_zmask, int bra_from_lam) {
-    __shared__ double acc[kMaxSlots][kBlock];
+    static_assert(kAccCols * kAccLanes == kBlock, "one accumulator column per lane pair");
+    __shared__ double acc[kMaxSlotsSv][kAccCols];
+    __shared__ double sink[kAccCols];
+    const AccCols cols{acc, sink};
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const CFOp *cops = (const CFOp *)ops;
     const CFGen *cgens = (const CFGen *)gens;
     const int nslots = pd->nslots;
-    for (int s = 0; s < nslots; ++s) acc[s][threadIdx.x] = 0.0;
+    // each column is cleared by the lane pair that adds to it (both lanes
+    // store the same zero)
+    for (int s = 0; s < nslots; ++s) acc[s][threadIdx.x >> 1] = 0.0;
     const uint64_t wave = xcd_block() * kWavesPerBlock + w;
     const uint64_t t0 = wave * chunk, t1 = min(t0 + chunk, ntiles);
     const int nops = pd->nops;
@@ -470,7 +478,7 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
         wait_op(cur);
         for (int o = 0; o < nops;) {
             if (f_kind(cur) == kDiagRun) {
-                diag_run_adj<R, NR, RB>(cops + o, cur, base, a, b, acc, threadIdx.x);
+                diag_run_adj<R, NR, RB>(cops + o, cur, base, a, b, cols);
                 o += 1 + run_ndata(cur) + run_nmem(cur);
                 cur = load_op(cops + (o < nops ? o : 0));
                 wait_op(cur);
@@ -478,26 +486,26 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
             }
             const FOp nxt = load_op(cops + (o + 1 < nops ? o + 1 : o));
             const int slot = f_slot(cur);
-            // overlaps go to the lane's own LDS column (no contention; the
+            // overlaps go to the lane pair's LDS column (no contention; the
             // additions keep program order)
             if (f_kind(cur) != kDiag && f_plain(cur)) {
                 if (slot >= 0) {
                     // X and Y generators need no generator record
                     FGen g{};
                     if (f_gkind(cur) == kGenDense) g = load_gen(cgens + slot);
-                    atomicAdd(&acc[slot][threadIdx.x], adj_plain_dense<R, NR, true>(cur, &g, lane, a, b));
+                    acc_add(cols, slot, adj_plain_dense<R, NR, true>(cur, &g, lane, a, b));
                 } else {
                     adj_plain_dense<R, NR, false>(cur, nullptr, lane, a, b);
                 }
             } else if (diag_fast(cur)) {
                 if (slot >= 0)
-                    atomicAdd(&acc[slot][threadIdx.x], adj_diag_step<R, NR, true>(cur, cur.m[2].re, cur.m[3].re, base, a, b));
+                    acc_add(cols, slot, adj_diag_step<R, NR, true>(cur, cur.m[2].re, cur.m[3].re, base, a, b));
                 else
                     adj_diag_step<R, NR, false>(cur, 0.0, 0.0, base, a, b);
             } else {
                 if (slot >= 0) {
                     const FGen g = f_kind(cur) == kDiag ? diag_gen(cur) : load_gen(cgens + slot);
-                    atomicAdd(&acc[slot][threadIdx.x], gen_dot<R, NR>(cur, g, lane, base, a, b));
+                    acc_add(cols, slot, gen_dot<R, NR>(cur, g, lane, base, a, b));
                 }
                 apply_fop<R, NR>(cur, lane, base, a);
                 apply_fop<R, NR>(cur, lane, base, b);
@@ -513,9 +521,9 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
         }
     }
     __syncthreads();
-    // slot s: wavefront w sums the 256 columns in a fixed order
+    // slot s: wavefront w sums the 128 columns in a fixed order
     for (int s = w; s < nslots; s += kWavesPerBlock) {
-        double v = acc[s][lane] + acc[s][lane + 64] + acc[s][lane + 128] + acc[s][lane + 192];
+        double v = acc[s][lane] + acc[s][lane + 64];
         v = wave_sum(v);
         if (lane == 0) partials[size_t(s) * gridDim.x + blockIdx.x] = v;
     }
@@ -1032,7 +1040,7 @@ namespace lgh {
 
 void fused_apply(lgh_sv *sv, const std::vector<KOp> &kops) {
     const FusedCfg cfg = fused_cfg(sv->device, sv->n, sv->dtype, 0);
-    const auto passes = schedule_passes_cached(kops, sv->n, {}, cfg.rb, fixed_lanes(sv->dtype));
+    const auto passes = schedule_passes_cached(kops, sv->n, {}, cfg.rb, fixed_lanes(sv->dtype), kMaxSlots);
     if (sv->dtype == LGH_C128)
         run_forward<double>(sv, kops, *passes, cfg);
     else
@@ -1057,7 +1065,8 @@ FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, 
         }
     }
     const int rb = fused_cfg(0, n, dtype, adjoint ? 1 : 0).rb;
-    const auto passes = schedule_passes_cached(kops, n, adjoint ? slot : std::vector<int>{}, rb, fixed_lanes(dtype));
+    const auto passes = schedule_passes_cached(kops, n, adjoint ? slot : std::vector<int>{}, rb, fixed_lanes(dtype),
+                                               kMaxSlotsSv);
     FusionPlan plan{};
     for (const Pass &p : *passes) {
         ++plan.passes;
@@ -1095,7 +1104,7 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
     const int n = lam->n;
     const FusedCfg cfg = fused_cfg(lam->device, n, lam->dtype, 1);
     const int rb = cfg.rb;
-    const auto passes = schedule_passes_cached(udag, n, slot_col, rb, fixed_lanes(lam->dtype));
+    const auto passes = schedule_passes_cached(udag, n, slot_col, rb, fixed_lanes(lam->dtype), kMaxSlotsSv);
     const Geometry geo = geometry(n, lam->device, rb, cfg.waves_per_cu, g_waves_adj.load());
     // lambda still to be read from lam_src (until the first pass has run)
     const void *src = lam_src;
@@ -1173,7 +1182,7 @@ void fused_adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std:
 size_t fused_partials_bytes(int n, int device) {
     // sized for the smaller tile, which has more workgroups
     if (n < kLaneBits + 3) return 0;
-    return sizeof(double) * size_t(kMaxSlots) * geometry(n, device, 3, kMaxWavesPerCUAdjoint).blocks;
+    return sizeof(double) * size_t(kMaxSlotsSv) * geometry(n, device, 3, kMaxWavesPerCUAdjoint).blocks;
 }
 
 } // namespace lgh

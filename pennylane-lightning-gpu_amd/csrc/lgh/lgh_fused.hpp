@@ -32,18 +32,20 @@ void set_pass_waves(int64_t forward, int64_t adjoint);
 // rb: register bits of the passes (tile = 2^(6 + rb) amplitudes per wavefront)
 bool fusable(const KOp &k, int n, int rb);
 // trainable_slot[i] >= 0 marks gates whose generator overlap is accumulated
-// (at most kMaxSlots per pass); empty for the forward direction.
+// (at most max_slots per pass: kMaxSlots for the batched kernels,
+// kMaxSlotsSv for the single-state adjoint kernel); empty for the forward
+// direction.
 // fixed_lanes: the lane positions 0..fixed_lanes-1 carry index bits
 // 0..fixed_lanes-1 in every pass; the scheduler chooses the index bits of the
 // other lane positions per pass, like the register bits (6: all lane bits
-// fixed).
+// fixed).  max_slots does not matter without trainable gates.
 std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std::vector<int> &trainable_slot,
-                                  int rb, int fixed_lanes = 6);
+                                  int rb, int fixed_lanes, int max_slots);
 // schedule_passes through a cache keyed by the gates' structure (the
 // schedule does not depend on parameter values).
 std::shared_ptr<const std::vector<Pass>> schedule_passes_cached(const std::vector<KOp> &ops, int n,
                                                                 const std::vector<int> &trainable_slot, int rb,
-                                                                int fixed_lanes);
+                                                                int fixed_lanes, int max_slots);
 // Host-side plan of the fused passes a gate sequence gets (no device use):
 // forward passes, or adjoint passes with every parametric gate trainable.
 struct FusionPlan {

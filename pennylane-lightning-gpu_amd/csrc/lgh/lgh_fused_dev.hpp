@@ -20,10 +20,16 @@ namespace fz {
 constexpr int kMaxRB = 4;
 constexpr int kMaxNR = 1 << kMaxRB;
 constexpr int kLaneBits = 6;
-// trainable gates per pass: the adjoint kernels keep one double per slot and
-// lane in LDS (kMaxSlots x 256 x 8 B = 48 KiB per workgroup, three
-// workgroups per CU)
+// Trainable gates per pass.  The adjoint kernels keep their overlap sums in
+// LDS, 48 KiB per workgroup of 256 lanes (three workgroups per CU): the
+// batched kernel one double per slot and lane (kMaxSlots), the single-state
+// kernel one per slot and pair of lanes (kMaxSlotsSv), the pair's two values
+// added with one lane exchange first.  A pass holds as many trainable gates as
+// it has slots, so twice the slots means fewer passes over the two states.
 constexpr int kMaxSlots = 24;
+constexpr int kAccLanes = 2;                     // lanes per accumulator column (single-state kernel)
+constexpr int kAccCols = 256 / kAccLanes;        // columns per workgroup
+constexpr int kMaxSlotsSv = kMaxSlots * kAccLanes;
 constexpr int kRBForward = 4;  // forward passes (one state)
 constexpr int kRBAdjoint = 3;  // adjoint passes (lambda and one observable state)
 
@@ -820,11 +826,28 @@ __device__ __forceinline__ void diag_run(const CFOp *hp, const FOp &hdr, uint64_
     run_finish<R, NR>(hp, flags, q, sm, v);
 }
 
+// Overlap sums of the single-state adjoint kernel: acc[slot][column], one
+// column per lane pair, and a sink row that takes the odd lanes' adds (never
+// read), so that adding stays free of divergent branches.
+struct AccCols {
+    double (*acc)[kAccCols];
+    double *sink;
+};
+// Adds a lane's overlap to its slot: the lane pair's values are summed first
+// and the even lane adds the sum to the pair's column (its own column: no
+// contention, program order).
+__device__ __forceinline__ void acc_add(const AccCols &c, int slot, double v) {
+    static_assert(kAccLanes == 2, "one exchange sums a lane pair");
+    const int lane = threadIdx.x & 63, col = threadIdx.x >> 1;
+    v += xor_lane<0>(v, lane);
+    atomicAdd((lane & 1) ? &c.sink[col] : &c.acc[slot][col], v);
+}
+
 // Adjoint: the run applied to lambda (a) and the observable state (b); the
 // overlap of every trainable member is added to its slot of acc.
 template <class R, int NR, int RB>
 __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, uint64_t base, cx<R> (&a)[NR],
-                                             cx<R> (&b)[NR], double (*acc)[256], int col) {
+                                             cx<R> (&b)[NR], const AccCols &acc) {
     const int nmem = run_nmem(hdr);
     const uint32_t flags = run_flags(hdr);
     const CFOp *mp = hp + 1 + run_ndata(hdr);
@@ -869,7 +892,7 @@ __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, uin
                 }
                 val = lane_ok ? val : 0.0;
             }
-            atomicAdd(&acc[slot][col], val); // own column: no contention, program order
+            acc_add(acc, slot, val);
         }
         switch (f_mclass(m)) {
         case kMemQ: q = run_q(m, base, q); break;

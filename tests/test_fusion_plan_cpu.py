@@ -21,7 +21,7 @@ def test_benchmark_circuit_plan():
     ops = ry_rz_cz_brick(28, 6, seed=1337)  # 498 gates: 168 RY, 168 RZ, 162 CZ
     ndiag = sum(1 for o in ops if o[0] in ("RZ", "CZ"))
     adj = plan(28, ops, adjoint=True)
-    assert adj["standalone"] == 0 and adj["passes"] <= 15
+    assert adj["standalone"] == 0 and adj["passes"] <= 11
     # every diagonal gate of the reverse sweep is a member of a diagonal run
     assert adj["run_members"] == ndiag
     fwd = plan(28, ops)
