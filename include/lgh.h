@@ -157,6 +157,8 @@ typedef struct lgh_tune_result {
     int32_t cand_tile_bits[16];
     int32_t cand_waves_per_cu[16];
     double cand_ms[16];
+    int32_t adjoint_slots;  /* trainable gates per adjoint pass of the chosen candidate */
+    int32_t cand_slots[16]; /* per candidate (0 for forward candidates) */
 } lgh_tune_result;
 lgh_status lgh_autotune_fused(int device, int32_t num_qubits, lgh_dtype dtype, const lgh_op *ops, int64_t num_ops,
                               lgh_tune_result *out);

@@ -70,6 +70,7 @@ constexpr int kMaxWavesPerCUAdjoint = 512;
 struct FusedCfg {
     int rb;
     int waves_per_cu;
+    int slots; // trainable gates per adjoint pass (<= kMaxSlotsSv); unused forward
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, FusedCfg> g_tuned;
@@ -89,7 +90,8 @@ FusedCfg fused_cfg(int device, int n, lgh_dtype dt, int dir) {
         auto it = g_tuned.find(std::make_tuple(device, n, int(dt), dir));
         if (it != g_tuned.end()) return it->second;
     }
-    return dir == 0 ? FusedCfg{g_rb_fwd.load(), kWavesPerCUForward} : FusedCfg{g_rb_adj.load(), kWavesPerCUAdjoint};
+    return dir == 0 ? FusedCfg{g_rb_fwd.load(), kWavesPerCUForward, kMaxSlots}
+                    : FusedCfg{g_rb_adj.load(), kWavesPerCUAdjoint, kMaxSlotsSv};
 }
 
 struct Use {
@@ -1064,9 +1066,10 @@ FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, 
             slot.push_back(-1);
         }
     }
-    const int rb = fused_cfg(0, n, dtype, adjoint ? 1 : 0).rb;
+    const FusedCfg cfg = fused_cfg(0, n, dtype, adjoint ? 1 : 0);
+    const int rb = cfg.rb;
     const auto passes = schedule_passes_cached(kops, n, adjoint ? slot : std::vector<int>{}, rb, fixed_lanes(dtype),
-                                               kMaxSlotsSv);
+                                               adjoint ? cfg.slots : kMaxSlots);
     FusionPlan plan{};
     for (const Pass &p : *passes) {
         ++plan.passes;
@@ -1104,7 +1107,7 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
     const int n = lam->n;
     const FusedCfg cfg = fused_cfg(lam->device, n, lam->dtype, 1);
     const int rb = cfg.rb;
-    const auto passes = schedule_passes_cached(udag, n, slot_col, rb, fixed_lanes(lam->dtype), kMaxSlotsSv);
+    const auto passes = schedule_passes_cached(udag, n, slot_col, rb, fixed_lanes(lam->dtype), cfg.slots);
     const Geometry geo = geometry(n, lam->device, rb, cfg.waves_per_cu, g_waves_adj.load());
     // lambda still to be read from lam_src (until the first pass has run)
     const void *src = lam_src;
@@ -1224,9 +1227,10 @@ extern "C" lgh_status lgh_fusion_plan(int32_t num_qubits, const lgh_op *ops, int
 // ---------------------------------------------------------------- autotuner
 //
 // Times the fused passes of one circuit under every candidate tile height
-// (register bits 3 and 4) and grid (wavefronts per CU) on states of the
-// requested size, forward and adjoint separately, and keeps the fastest per
-// (device, qubits, dtype, direction).  Results do not depend on the grid;
+// (register bits 3 and 4), grid (wavefronts per CU) and, for the adjoint,
+// number of trainable gates per pass, on states of the requested size,
+// forward and adjoint separately, and keeps the fastest per (device, qubits,
+// dtype, direction).  Results do not depend on the grid;
 // other tile heights schedule the gates into other passes, which changes the
 // rounding of the results but not their values beyond it.
 
@@ -1332,8 +1336,13 @@ extern "C" lgh_status lgh_autotune_fused(int device, int32_t num_qubits, lgh_dty
         const int32_t zw = 0;
         const int64_t ncirc = int64_t(circ.size());
 
-        const FusedCfg fwd_c[] = {{3, 512}, {3, 4096}, {4, 512}, {4, 4096}};
-        const FusedCfg adj_c[] = {{3, 128}, {3, 256}, {3, 512}, {4, 128}, {4, 256}, {4, 512}};
+        // forward: tile height x grid; adjoint: tile height x grid x trainable
+        // gates per pass (more per pass: fewer passes over the two states,
+        // more work per pass)
+        const FusedCfg fwd_c[] = {{3, 512, kMaxSlots}, {3, 4096, kMaxSlots}, {4, 512, kMaxSlots}, {4, 4096, kMaxSlots}};
+        const FusedCfg adj_c[] = {{3, 128, kMaxSlotsSv}, {3, 256, kMaxSlotsSv}, {3, 512, kMaxSlotsSv},
+                                  {3, 256, kMaxSlotsSv / 2}, {3, 512, kMaxSlotsSv / 2}, {4, 128, kMaxSlotsSv},
+                                  {4, 256, kMaxSlotsSv}, {4, 512, kMaxSlotsSv}};
         const FusedCfg def_f = fused_cfg(device, n, dtype, 0), def_a = fused_cfg(device, n, dtype, 1);
         FusedCfg best_f = def_f, best_a = def_a;
         double tbest_f = 1e300, tbest_a = 1e300;
@@ -1345,6 +1354,7 @@ extern "C" lgh_status lgh_autotune_fused(int device, int32_t num_qubits, lgh_dty
             out->cand_dir[q] = 0;
             out->cand_tile_bits[q] = c.rb;
             out->cand_waves_per_cu[q] = c.waves_per_cu;
+            out->cand_slots[q] = 0;
             out->cand_ms[q] = ms;
             if (c.rb == def_f.rb && c.waves_per_cu == def_f.waves_per_cu) out->default_forward_ms = ms;
             if (ms < tbest_f) {
@@ -1372,8 +1382,10 @@ extern "C" lgh_status lgh_autotune_fused(int device, int32_t num_qubits, lgh_dty
             out->cand_dir[q] = 1;
             out->cand_tile_bits[q] = c.rb;
             out->cand_waves_per_cu[q] = c.waves_per_cu;
+            out->cand_slots[q] = c.slots;
             out->cand_ms[q] = ms;
-            if (c.rb == def_a.rb && c.waves_per_cu == def_a.waves_per_cu) out->default_adjoint_ms = ms;
+            if (c.rb == def_a.rb && c.waves_per_cu == def_a.waves_per_cu && c.slots == def_a.slots)
+                out->default_adjoint_ms = ms;
             if (ms < tbest_a) {
                 tbest_a = ms;
                 best_a = c;
@@ -1388,6 +1400,7 @@ extern "C" lgh_status lgh_autotune_fused(int device, int32_t num_qubits, lgh_dty
         out->forward_waves_per_cu = best_f.waves_per_cu;
         out->adjoint_tile_bits = best_a.rb;
         out->adjoint_waves_per_cu = best_a.waves_per_cu;
+        out->adjoint_slots = best_a.slots;
         out->forward_ms = tbest_f;
         out->adjoint_ms = tbest_a;
     });

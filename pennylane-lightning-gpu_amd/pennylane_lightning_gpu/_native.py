@@ -62,7 +62,8 @@ class TuneResult(C.Structure):
                 ("forward_ms", C.c_double), ("adjoint_ms", C.c_double),
                 ("default_forward_ms", C.c_double), ("default_adjoint_ms", C.c_double),
                 ("candidates", C.c_int32), ("cand_dir", C.c_int32 * 16), ("cand_tile_bits", C.c_int32 * 16),
-                ("cand_waves_per_cu", C.c_int32 * 16), ("cand_ms", C.c_double * 16)]
+                ("cand_waves_per_cu", C.c_int32 * 16), ("cand_ms", C.c_double * 16),
+                ("adjoint_slots", C.c_int32), ("cand_slots", C.c_int32 * 16)]
 
 
 _lib = None
@@ -203,10 +204,14 @@ def autotune_fused(num_qubits, ops=None, dtype=np.complex128, device=0):
         check(lib().lgh_autotune_fused(int(device), int(num_qubits), code, None, 0, C.byref(r)))
     cands = [{"dir": "forward" if r.cand_dir[q] == 0 else "adjoint", "tile_bits": r.cand_tile_bits[q],
               "waves_per_cu": r.cand_waves_per_cu[q], "ms": round(r.cand_ms[q], 4)} for q in range(r.candidates)]
+    for q, c in enumerate(cands):
+        if c["dir"] == "adjoint":
+            c["slots"] = r.cand_slots[q]
     return {"forward": {"tile_bits": r.forward_tile_bits, "waves_per_cu": r.forward_waves_per_cu,
                         "ms": round(r.forward_ms, 4), "default_ms": round(r.default_forward_ms, 4)},
             "adjoint": {"tile_bits": r.adjoint_tile_bits, "waves_per_cu": r.adjoint_waves_per_cu,
-                        "ms": round(r.adjoint_ms, 4), "default_ms": round(r.default_adjoint_ms, 4)},
+                        "slots": r.adjoint_slots, "ms": round(r.adjoint_ms, 4),
+                        "default_ms": round(r.default_adjoint_ms, 4)},
             "candidates": cands}
 
 

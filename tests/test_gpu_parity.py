@@ -715,9 +715,10 @@ def test_fused_deep_single_pass_rotation_scale(dtype):
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 def test_autotuned_fused_passes_match_oracle(dtype):
-    """lgh_autotune_fused times every candidate tile height and grid on a
-    16-qubit circuit and keeps the fastest; forward states and adjoint
-    Jacobians computed under the tuned choice still match the oracle."""
+    """lgh_autotune_fused times every candidate tile height and grid (and,
+    for the adjoint, trainable gates per pass) on a 16-qubit circuit and
+    keeps the fastest; forward states and adjoint Jacobians computed under
+    the tuned choice still match the oracle."""
     n = 16
     rng = np.random.default_rng(1616)
     ops = []
@@ -733,7 +734,9 @@ def test_autotuned_fused_passes_match_oracle(dtype):
         assert res["forward"]["tile_bits"] in (3, 4) and res["adjoint"]["tile_bits"] in (3, 4)
         assert res["forward"]["waves_per_cu"] in (512, 4096)
         assert res["adjoint"]["waves_per_cu"] in (128, 256, 512)
-        assert len(res["candidates"]) == 10
+        assert res["adjoint"]["slots"] in (24, 48)
+        assert len(res["candidates"]) == 12
+        assert {c["slots"] for c in res["candidates"] if c["dir"] == "adjoint"} == {24, 48}
         assert res["forward"]["ms"] <= res["forward"]["default_ms"] + 1e-9
         assert res["adjoint"]["ms"] <= res["adjoint"]["default_ms"] + 1e-9
         psi = N.State(n, dtype).reset().apply(ops)
