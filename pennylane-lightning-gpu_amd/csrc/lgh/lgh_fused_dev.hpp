@@ -563,12 +563,39 @@ __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, co
     } else {
         const bool hi = (lane >> L) & 1;
         const cx<R> cs = hi ? m.m11 : m.m00, cp = hi ? m.m10 : m.m01;
+        auto step = [&](int k, const cx<R> &pa) {
+            a[k] = row2<R, Real, Rot == 1 ? 1 : 0>(cs, cp, a[k], pa);
+            b[k] = row2<R, Real, Rot == 1 ? 1 : 0>(cs, cp, b[k], xor_lane_cx<L>(b[k], lane));
+        };
+        if constexpr (Gen) {
+            // X and Y generators: one chain of two FMAs per register; the Y
+            // row's sign depends only on the lane and is applied once
+            if (gk == kGenY) {
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    const cx<R> pa = xor_lane_cx<L>(a[k], lane);
+                    s = fma(double(b[k].re), double(pa.re), s);
+                    s = fma(double(b[k].im), double(pa.im), s);
+                    step(k, pa);
+                }
+                return hi ? s : -s;
+            }
+            if (gk == kGenX) {
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    const cx<R> pa = xor_lane_cx<L>(a[k], lane);
+                    s = fma(double(b[k].re), double(pa.im), s);
+                    s = fma(-double(b[k].im), double(pa.re), s);
+                    step(k, pa);
+                }
+                return s;
+            }
+        }
 #pragma unroll
         for (int k = 0; k < NR; ++k) {
             const cx<R> pa = xor_lane_cx<L>(a[k], lane);
             if constexpr (Gen) s += row_dot<R>(gk, *g, hi, a[k], pa, b[k]);
-            a[k] = row2<R, Real, Rot == 1 ? 1 : 0>(cs, cp, a[k], pa);
-            b[k] = row2<R, Real, Rot == 1 ? 1 : 0>(cs, cp, b[k], xor_lane_cx<L>(b[k], lane));
+            step(k, pa);
         }
     }
     return s;
