@@ -764,7 +764,8 @@ __device__ __forceinline__ FGen diag_gen(const FOp &op) {
 // --- diagonal runs ---------------------------------------------------------
 // per-lane factor of a member without register bits
 template <class R> __device__ __forceinline__ cx<R> run_q(const FOp &m, uint64_t base, cx<R> q) {
-    const bool lane_ok = (base & m.cm_rest) == m.cm_rest;
+    bool lane_ok = true;
+    if (m.cm_rest) lane_ok = (base & m.cm_rest) == m.cm_rest; // uniform test first
     if (f_dneg(m)) {
         const uint32_t fm = lane_ok ? kSignBit : 0u;
         return cx<R>{flip(q.re, fm), flip(q.im, fm)};
@@ -900,24 +901,54 @@ __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, uin
         const int slot = f_slot(m);
         if (slot >= 0) {
             const uint32_t oc = f_oclass(m);
-            const bool lane_ok = (base & m.cm_rest) == m.cm_rest;
             double val;
-            if (oc == kOvF) {
-                val = diag_dot<R, NR>(m, diag_gen(m), base, lane_ok, a, b);
-            } else {
+            if (oc == kOvB && m.pm_rest == 0 && m.cm_rest == 0) {
+                // a phase on register bits only (RZ on a register bit): the
+                // registers with the bit clear take g_even, the others g_odd
                 const double ge = m.m[2].re, go = m.m[3].re;
-                const bool lpar = __popcll(base & m.pm_rest) & 1;
-                const double g0 = lpar ? go : ge, g1 = lpar ? ge : go;
-                if (oc == kOvT) {
-                    val = g0 * T;
-                } else {
-                    double e = E[0];
-#pragma unroll
-                    for (int r = 1; r < RB; ++r)
-                        if (f_loc(m) == r) e = E[r];
-                    val = g0 * e + g1 * (T - e);
+                const double gt = go * T, d = ge - go;
+                switch (f_loc(m)) { // uniform: each case reads its own E
+                case 0: val = fma(d, E[0], gt); break;
+                case 1:
+                    if constexpr (RB > 1) {
+                        val = fma(d, E[1], gt);
+                        break;
+                    }
+                    [[fallthrough]];
+                case 2:
+                    if constexpr (RB > 2) {
+                        val = fma(d, E[2], gt);
+                        break;
+                    }
+                    [[fallthrough]];
+                default:
+                    if constexpr (RB > 3) {
+                        val = fma(d, E[RB - 1], gt);
+                        break;
+                    }
+                    val = fma(d, E[0], gt);
+                    break;
                 }
-                val = lane_ok ? val : 0.0;
+            } else {
+                bool lane_ok = true;
+                if (m.cm_rest) lane_ok = (base & m.cm_rest) == m.cm_rest;
+                if (oc == kOvF) {
+                    val = diag_dot<R, NR>(m, diag_gen(m), base, lane_ok, a, b);
+                } else {
+                    const double ge = m.m[2].re, go = m.m[3].re;
+                    const bool lpar = __popcll(base & m.pm_rest) & 1;
+                    const double g0 = lpar ? go : ge, g1 = lpar ? ge : go;
+                    if (oc == kOvT) {
+                        val = g0 * T;
+                    } else {
+                        double e = E[0];
+#pragma unroll
+                        for (int r = 1; r < RB; ++r)
+                            if (f_loc(m) == r) e = E[r];
+                        val = g0 * e + g1 * (T - e);
+                    }
+                    val = lane_ok ? val : 0.0;
+                }
             }
             acc_add(acc, slot, val);
         }
