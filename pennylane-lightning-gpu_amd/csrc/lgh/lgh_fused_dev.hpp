@@ -475,6 +475,13 @@ __device__ __forceinline__ double pair_dot_acc(uint32_t gk, const FGen &g, cx<R>
     }
     return s + pair_dot<R>(gk, g, a0, a1, b0, b1);
 }
+// the same with the generator kind fixed at compile time (GK < 0: read gk)
+template <int GK, class R>
+__device__ __forceinline__ double pair_dot_acc_k(uint32_t gk, const FGen &g, cx<R> a0, cx<R> a1, cx<R> b0,
+                                                 cx<R> b1, double s) {
+    if constexpr (GK >= 0) gk = uint32_t(GK);
+    return pair_dot_acc<R>(gk, g, a0, a1, b0, b1, s);
+}
 // lane bit L <= 3: one lane's row of the pair sum; own = its amplitude,
 // partner = the amplitude in lane ^ 2^L, hi = the lane's bit L
 template <class R>
@@ -531,7 +538,8 @@ __device__ __forceinline__ double lane_dot_masked(const FOp &op, const FGen &g, 
 // One reverse step of an uncontrolled dense gate on (lambda, bra): the
 // generator overlap (with Gen) and U^dagger on both states, sharing the lane
 // exchanges between them.
-template <class R, int NR, int B, bool Real, bool Gen, int Rot = 0>
+// GK >= 0: the generator kind is known at compile time.
+template <class R, int NR, int B, bool Real, bool Gen, int Rot = 0, int GK = -1>
 __device__ __forceinline__ double adj_reg_plain(const M2<R> &m, uint32_t gk, const FGen *g, cx<R> (&a)[NR],
                                                 cx<R> (&b)[NR]) {
     double s = 0.0;
@@ -539,13 +547,13 @@ __device__ __forceinline__ double adj_reg_plain(const M2<R> &m, uint32_t gk, con
     for (int k = 0; k < NR; ++k) {
         if (k & (1 << B)) continue;
         const int k1 = k | (1 << B);
-        if constexpr (Gen) s = pair_dot_acc<R>(gk, *g, a[k], a[k1], b[k], b[k1], s);
+        if constexpr (Gen) s = pair_dot_acc_k<GK, R>(gk, *g, a[k], a[k1], b[k], b[k1], s);
         mat2<R, Real, Rot>(m, a[k], a[k1]);
         mat2<R, Real, Rot>(m, b[k], b[k1]);
     }
     return s;
 }
-template <class R, int NR, int L, bool Real, bool Gen, int Rot = 0>
+template <class R, int NR, int L, bool Real, bool Gen, int Rot = 0, int GK = -1>
 __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, const FGen *g, int lane,
                                                  cx<R> (&a)[NR], cx<R> (&b)[NR]) {
     double s = 0.0;
@@ -554,7 +562,7 @@ __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, co
         for (int k = 0; k < NR; k += 2) {
             row_swap_cx<L>(a[k], a[k + 1]);
             row_swap_cx<L>(b[k], b[k + 1]);
-            if constexpr (Gen) s = pair_dot_acc<R>(gk, *g, a[k], a[k + 1], b[k], b[k + 1], s);
+            if constexpr (Gen) s = pair_dot_acc_k<GK, R>(gk, *g, a[k], a[k + 1], b[k], b[k + 1], s);
             mat2<R, Real, Rot>(m, a[k], a[k + 1]);
             mat2<R, Real, Rot>(m, b[k], b[k + 1]);
             row_swap_cx<L>(a[k], a[k + 1]);
@@ -605,17 +613,32 @@ __device__ __forceinline__ double adj_lane_plain(const M2<R> &m, uint32_t gk, co
 template <class R, int NR, int B, bool Gen>
 __device__ __forceinline__ double adj_reg_any(const M2<R> &m, int rot, bool real, uint32_t gk, const FGen *g,
                                               cx<R> (&a)[NR], cx<R> (&b)[NR]) {
-    if (rot == 1) return adj_reg_plain<R, NR, B, true, Gen, 1>(m, gk, g, a, b);
-    if (rot == 2) return adj_reg_plain<R, NR, B, true, Gen, 2>(m, gk, g, a, b);
+    // the rotation forms of RY (Y generator) get the overlap without a
+    // per-pair test of the generator kind
+    if (rot == 1) {
+        if (Gen && gk == kGenY) return adj_reg_plain<R, NR, B, true, Gen, 1, kGenY>(m, gk, g, a, b);
+        return adj_reg_plain<R, NR, B, true, Gen, 1>(m, gk, g, a, b);
+    }
+    if (rot == 2) {
+        if (Gen && gk == kGenY) return adj_reg_plain<R, NR, B, true, Gen, 2, kGenY>(m, gk, g, a, b);
+        return adj_reg_plain<R, NR, B, true, Gen, 2>(m, gk, g, a, b);
+    }
     if (real) return adj_reg_plain<R, NR, B, true, Gen>(m, gk, g, a, b);
     return adj_reg_plain<R, NR, B, false, Gen>(m, gk, g, a, b);
 }
 template <class R, int NR, int L, bool Gen>
 __device__ __forceinline__ double adj_lane_any(const M2<R> &m, int rot, bool real, uint32_t gk, const FGen *g,
                                                int lane, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
-    if (rot == 1) return adj_lane_plain<R, NR, L, true, Gen, 1>(m, gk, g, lane, a, b);
+    if (rot == 1) {
+        if constexpr (L >= 4)
+            if (Gen && gk == kGenY) return adj_lane_plain<R, NR, L, true, Gen, 1, kGenY>(m, gk, g, lane, a, b);
+        return adj_lane_plain<R, NR, L, true, Gen, 1>(m, gk, g, lane, a, b);
+    }
     if constexpr (L >= 4)
-        if (rot == 2) return adj_lane_plain<R, NR, L, true, Gen, 2>(m, gk, g, lane, a, b);
+        if (rot == 2) {
+            if (Gen && gk == kGenY) return adj_lane_plain<R, NR, L, true, Gen, 2, kGenY>(m, gk, g, lane, a, b);
+            return adj_lane_plain<R, NR, L, true, Gen, 2>(m, gk, g, lane, a, b);
+        }
     if (real) return adj_lane_plain<R, NR, L, true, Gen>(m, gk, g, lane, a, b);
     return adj_lane_plain<R, NR, L, false, Gen>(m, gk, g, lane, a, b);
 }
