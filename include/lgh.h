@@ -137,6 +137,12 @@ lgh_status lgh_set_tile_bits(int forward, int adjoint);
  * amplitudes, so a small cap exercises the multi-tile path at any size.  No
  * counterpart in the reference; a tuning knob used by tests. */
 lgh_status lgh_set_pass_waves(int64_t forward, int64_t adjoint);
+/* Lean fused passes (on by default): a pass whose dense gates are all
+ * uncontrolled real 2x2 matrices and whose diagonal gates all sit in
+ * diagonal runs without general members runs in kernels compiled for just
+ * those forms.  Results agree with the general kernels within rounding.  No
+ * counterpart in the reference; a switch for comparisons and tests. */
+lgh_status lgh_set_lean_passes(int enabled);
 /* Autotuner of the fused passes (no counterpart in the reference): times
  * `ops` (or, with num_ops = 0, two layers of RY and RZ on every wire plus a
  * CZ brick) on fresh states of num_qubits (10..34) qubits under every

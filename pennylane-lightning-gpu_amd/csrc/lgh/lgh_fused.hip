@@ -49,6 +49,8 @@ std::atomic<int> g_fusion{1};
 // floats), whole cache lines; the other lane positions are chosen per pass.
 int fixed_lanes(lgh_dtype dt) { return dt == LGH_C128 ? 3 : 4; }
 std::atomic<int> g_rb_fwd{kRBForward}, g_rb_adj{kRBAdjoint};
+// passes with only lean gate forms run in the lean kernels (lgh_set_lean_passes)
+std::atomic<bool> g_lean{true};
 // caps on the wavefronts of a pass's grid (0: the per-CU defaults below)
 std::atomic<int64_t> g_waves_fwd{0}, g_waves_adj{0};
 
@@ -368,7 +370,8 @@ using namespace lgh;
 namespace {
 
 // Forward pass: every wavefront processes tiles [w*chunk, (w+1)*chunk).
-template <class R, int RB>
+// Lean: the pass holds only the gate forms of a lean pass (lean_pass).
+template <class R, int RB, bool Lean = false>
 __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, uint64_t ntiles, uint64_t chunk,
                                                        const PassDesc *__restrict__ pd,
                                                        const FOp *__restrict__ ops) {
@@ -397,14 +400,17 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
         wait_op(cur);
         for (int o = 0; o < nops;) {
             if (f_kind(cur) == kDiagRun) {
-                diag_run<R, NR>(cops + o, cur, base, v);
+                diag_run<R, NR, Lean>(cops + o, cur, base, v);
                 o += 1 + run_ndata(cur) + run_nmem(cur);
                 cur = load_op(cops + (o < nops ? o : 0));
                 wait_op(cur);
                 continue;
             }
             const FOp nxt = load_op(cops + (o + 1 < nops ? o + 1 : o)); // prefetch the next descriptor
-            apply_fop<R, NR>(cur, lane, base, v);
+            if constexpr (Lean)
+                apply_lean<R, NR>(cur, lane, v);
+            else
+                apply_fop<R, NR>(cur, lane, base, v);
             cur = nxt;
             ++o;
         }
@@ -428,7 +434,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
 // (-1)^popcount(index & bra_zmask) per amplitude), then written like any
 // pass's output: the first reverse pass of a sweep with a Z-word observable
 // needs no separate pass that forms it.
-template <class R, int RB>
+template <class R, int RB, bool Lean>
 __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                const FOp *__restrict__ ops, const FGen *__restrict__ gens,
@@ -480,7 +486,7 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
         wait_op(cur);
         for (int o = 0; o < nops;) {
             if (f_kind(cur) == kDiagRun) {
-                diag_run_adj<R, NR, RB>(cops + o, cur, base, a, b, cols);
+                diag_run_adj<R, NR, RB, Lean>(cops + o, cur, base, a, b, cols);
                 o += 1 + run_ndata(cur) + run_nmem(cur);
                 cur = load_op(cops + (o < nops ? o : 0));
                 wait_op(cur);
@@ -490,7 +496,12 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
             const int slot = f_slot(cur);
             // overlaps go to the lane pair's LDS column (no contention; the
             // additions keep program order)
-            if (f_kind(cur) != kDiag && f_plain(cur)) {
+            if constexpr (Lean) {
+                if (slot >= 0)
+                    acc_add(cols, slot, adj_plain_lean<R, NR, true>(cur, lane, a, b));
+                else
+                    adj_plain_lean<R, NR, false>(cur, lane, a, b);
+            } else if (f_kind(cur) != kDiag && f_plain(cur)) {
                 if (slot >= 0) {
                     // X and Y generators need no generator record
                     FGen g{};
@@ -533,21 +544,22 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
 
 // Three register bits fit three wavefronts per SIMD (168 VGPRs, 48 KiB LDS
 // per workgroup); four register bits need all registers of one.
-template <class R>
+// Lean: the pass holds only the gate forms of a lean pass (lean_pass).
+template <class R, bool Lean = false>
 __global__ __launch_bounds__(kBlock, 3) void k_fused_adj3(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                        uint64_t ntiles, uint64_t chunk,
                                                        const PassDesc *__restrict__ pd, const FOp *__restrict__ ops,
                                                        const FGen *__restrict__ gens, double *__restrict__ partials,
                                                        uint64_t bra_zmask, int bra_from_lam) {
-    fused_adj_body<R, 3>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
+    fused_adj_body<R, 3, Lean>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
 }
-template <class R>
+template <class R, bool Lean = false>
 __global__ __launch_bounds__(kBlock) void k_fused_adj4(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                     uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                     const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                     double *__restrict__ partials, uint64_t bra_zmask,
                                                     int bra_from_lam) {
-    fused_adj_body<R, 4>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
+    fused_adj_body<R, 4, Lean>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
 }
 
 // slot sums -> jac[col[s]] * factor[s]
@@ -1028,7 +1040,9 @@ void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pas
         std::memcpy(blob.data() + pb, b.ops.data(), b.ops.size() * sizeof(FOp));
         unsigned char *d = static_cast<unsigned char *>(ctx->upload(blob.data(), blob.size()));
         ProfScope prof(kProfFusedPass, ctx->stream);
-        auto kern = rb == 4 ? k_fused_pass<R, 4> : k_fused_pass<R, 3>;
+        const bool lean = g_lean.load() && lean_pass(b.ops, false);
+        auto kern = rb == 4 ? (lean ? k_fused_pass<R, 4, true> : k_fused_pass<R, 4>)
+                            : (lean ? k_fused_pass<R, 3, true> : k_fused_pass<R, 3>);
         hipLaunchKernelGGL(kern, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream, static_cast<cx<R> *>(sv->data),
                            geo.ntiles, geo.chunk, reinterpret_cast<const PassDesc *>(d),
                            reinterpret_cast<const FOp *>(d + pb));
@@ -1150,7 +1164,9 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
         unsigned char *d = static_cast<unsigned char *>(ctx->upload(blob.data(), blob.size()));
         {
             ProfScope prof(kProfFusedAdj, ctx->stream);
-            auto kern = rb == 4 ? k_fused_adj4<R> : k_fused_adj3<R>;
+            const bool lean = g_lean.load() && lean_pass(b.ops, true);
+            auto kern = rb == 4 ? (lean ? k_fused_adj4<R, true> : k_fused_adj4<R>)
+                                : (lean ? k_fused_adj3<R, true> : k_fused_adj3<R>);
             const cx<R> *lam_in = static_cast<const cx<R> *>(src ? src : lam->data);
             src = nullptr;
             const uint64_t zmask = bra_pending ? bra_z->zmask : 0ull;
@@ -1200,6 +1216,10 @@ extern "C" lgh_status lgh_set_tile_bits(int forward, int adjoint) {
 
 extern "C" lgh_status lgh_set_pass_waves(int64_t forward, int64_t adjoint) {
     return guard([&] { set_pass_waves(forward, adjoint); });
+}
+
+extern "C" lgh_status lgh_set_lean_passes(int enabled) {
+    return guard([&] { g_lean.store(enabled != 0); });
 }
 
 extern "C" lgh_status lgh_fusion_plan(int32_t num_qubits, const lgh_op *ops, int64_t num_ops, int32_t adjoint,

@@ -671,6 +671,62 @@ __device__ __forceinline__ double adj_plain_dense(const FOp &op, const FGen *g, 
     }
 }
 
+// Adjoint: a lean pass's dense gate (uncontrolled, real: rotation form 1 or
+// 2 or a plain real matrix; trainable ones with an X or Y generator).
+template <class R, int NR, int B, bool Gen, int Rot>
+__device__ __forceinline__ double adj_reg_lean(const M2<R> &m, uint32_t gk, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
+    const FGen g{};
+    if constexpr (Gen) {
+        if (gk == kGenY) return adj_reg_plain<R, NR, B, true, true, Rot, kGenY>(m, gk, &g, a, b);
+        return adj_reg_plain<R, NR, B, true, true, Rot, kGenX>(m, gk, &g, a, b);
+    }
+    return adj_reg_plain<R, NR, B, true, false, Rot>(m, gk, &g, a, b);
+}
+template <class R, int NR, int L, bool Gen, int Rot>
+__device__ __forceinline__ double adj_lane_lean(const M2<R> &m, uint32_t gk, int lane, cx<R> (&a)[NR],
+                                                cx<R> (&b)[NR]) {
+    const FGen g{};
+    if constexpr (L >= 4 && Gen) {
+        if (gk == kGenY) return adj_lane_plain<R, NR, L, true, true, Rot, kGenY>(m, gk, &g, lane, a, b);
+        return adj_lane_plain<R, NR, L, true, true, Rot, kGenX>(m, gk, &g, lane, a, b);
+    }
+    // lane bits 0-3 take form 2 through the real path
+    return adj_lane_plain<R, NR, L, true, Gen, (L < 4 && Rot == 2) ? 0 : Rot>(m, gk, &g, lane, a, b);
+}
+template <class R, int NR, bool Gen>
+__device__ __forceinline__ double adj_plain_lean(const FOp &op, int lane, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
+    const M2<R> m = m2<R>(op);
+    const uint32_t gk = f_gkind(op);
+    const int rot = f_rot(op), loc = f_loc(op);
+#define LGH_REG3(B)                                                                                                    \
+    (rot == 1 ? adj_reg_lean<R, NR, B, Gen, 1>(m, gk, a, b)                                                          \
+              : rot == 2 ? adj_reg_lean<R, NR, B, Gen, 2>(m, gk, a, b) : adj_reg_lean<R, NR, B, Gen, 0>(m, gk, a, b))
+#define LGH_LANE3(L)                                                                                                   \
+    (rot == 1 ? adj_lane_lean<R, NR, L, Gen, 1>(m, gk, lane, a, b)                                                   \
+              : rot == 2 ? adj_lane_lean<R, NR, L, Gen, 2>(m, gk, lane, a, b)                                        \
+                         : adj_lane_lean<R, NR, L, Gen, 0>(m, gk, lane, a, b))
+    if (f_kind(op) == kRegDense) {
+        switch (loc) {
+        case 0: return LGH_REG3(0);
+        case 1: return LGH_REG3(1);
+        case 2: return LGH_REG3(2);
+        default:
+            if constexpr (NR > 8) return LGH_REG3(3);
+            return 0.0;
+        }
+    }
+    switch (loc) {
+    case 0: return LGH_LANE3(0);
+    case 1: return LGH_LANE3(1);
+    case 2: return LGH_LANE3(2);
+    case 3: return LGH_LANE3(3);
+    case 4: return LGH_LANE3(4);
+    default: return LGH_LANE3(5);
+    }
+#undef LGH_REG3
+#undef LGH_LANE3
+}
+
 // One reverse step of an uncontrolled RZ-type diagonal gate on (lambda, bra):
 // the overlap with a diagonal generator (eigenvalue g_even / g_odd by the
 // parity of the mask bits) and the phase on both states, the lane parity
@@ -784,6 +840,93 @@ __device__ __forceinline__ FGen diag_gen(const FOp &op) {
     return g;
 }
 
+// --- lean passes -------------------------------------------------------------
+// A pass whose dense gates are all uncontrolled real 2x2 matrices (the two
+// rotation forms or a plain real matrix) and whose diagonal gates are all
+// members of diagonal runs without general members runs in kernels compiled
+// with only those cases.  With the controlled and complex cases compiled in,
+// the compiler keeps the loop-carried tile in one set of registers and each
+// gate's results in another, and copies the whole tile back after every
+// gate; without them the gates update the tile where it lies.
+// host test: is a built op allowed in a lean pass
+inline bool lean_op(const FOp &f, bool adjoint) {
+    const uint32_t kind = f.meta & 15u;
+    if (kind == kDiagRun) return true; // members are checked separately
+    if (kind != kRegDense && kind != kLaneDense) return false;
+    if ((f.rbits & 0xffffu) != 0xffffu || f.cm_rest != 0) return false; // controls
+    const int rot = int((f.meta >> 28) & 3u);
+    const bool real = (f.meta >> 21) & 1u;
+    if (rot == 0 && !real) return false;
+    if (adjoint && int((f.meta >> 8) & 255u) > 0) { // trainable: X or Y generator
+        const uint32_t gk = (f.meta >> 16) & 15u;
+        if (gk != kGenX && gk != kGenY) return false;
+    }
+    return true;
+}
+inline bool lean_member(const FOp &m, bool adjoint) {
+    const uint32_t mc = (m.meta >> 24) & 3u;
+    if (mc == kMemX) return false;
+    if (adjoint && int((m.meta >> 8) & 255u) > 0 && ((m.meta >> 26) & 3u) == kOvF) return false;
+    return true;
+}
+inline bool lean_pass(const std::vector<FOp> &ops, bool adjoint) {
+    for (size_t j = 0; j < ops.size();) {
+        const FOp &f = ops[j];
+        if (!lean_op(f, adjoint)) return false;
+        if ((f.meta & 15u) == kDiagRun) {
+            const size_t nm = size_t(f.pad & 0xffffu), nd = size_t((f.pad >> 16) & 0xffu);
+            for (size_t q = 0; q < nm; ++q)
+                if (!lean_member(ops[j + 1 + nd + q], adjoint)) return false;
+            j += 1 + nd + nm;
+        } else {
+            ++j;
+        }
+    }
+    return true;
+}
+
+// forward: a lean pass's dense gate
+template <class R, int NR>
+__device__ __forceinline__ void apply_lean(const FOp &op, int lane, cx<R> (&v)[NR]) {
+    const M2<R> m = m2<R>(op);
+    const int loc = f_loc(op), rot = f_rot(op);
+    if (f_kind(op) == kRegDense) {
+#define LGH_REG_ARM(B)                                                                                                 \
+    if (rot == 1)                                                                                                      \
+        reg_plain<R, NR, B, true, 1>(m, v);                                                                            \
+    else if (rot == 2)                                                                                                 \
+        reg_plain<R, NR, B, true, 2>(m, v);                                                                            \
+    else                                                                                                               \
+        reg_plain<R, NR, B, true>(m, v);
+        switch (loc) {
+        case 0: LGH_REG_ARM(0) break;
+        case 1: LGH_REG_ARM(1) break;
+        case 2: LGH_REG_ARM(2) break;
+        default:
+            if constexpr (NR > 8) { LGH_REG_ARM(3) }
+            break;
+        }
+#undef LGH_REG_ARM
+        return;
+    }
+#define LGH_LANE_ARM(L)                                                                                                \
+    if (rot == 1)                                                                                                      \
+        lane_plain<R, NR, L, true, 1>(m, lane, v);                                                                     \
+    else if (L >= 4 && rot == 2)                                                                                       \
+        lane_plain<R, NR, L, true, 2>(m, lane, v);                                                                     \
+    else                                                                                                               \
+        lane_plain<R, NR, L, true>(m, lane, v);
+    switch (loc) {
+    case 0: LGH_LANE_ARM(0) break;
+    case 1: LGH_LANE_ARM(1) break;
+    case 2: LGH_LANE_ARM(2) break;
+    case 3: LGH_LANE_ARM(3) break;
+    case 4: LGH_LANE_ARM(4) break;
+    default: LGH_LANE_ARM(5) break;
+    }
+#undef LGH_LANE_ARM
+}
+
 // --- diagonal runs ---------------------------------------------------------
 // per-lane factor of a member without register bits
 template <class R> __device__ __forceinline__ cx<R> run_q(const FOp &m, uint64_t base, cx<R> q) {
@@ -854,8 +997,8 @@ __device__ __forceinline__ void run_finish2(const CFOp *hp, uint32_t flags, cx<R
         run_finish2_q<R, NR, false>(hp, flags, q, sm, a, b);
 }
 
-// Forward: the run whose header hdr is at hp.
-template <class R, int NR>
+// Forward: the run whose header hdr is at hp (Lean: no general members).
+template <class R, int NR, bool Lean = false>
 __device__ __forceinline__ void diag_run(const CFOp *hp, const FOp &hdr, uint64_t base, cx<R> (&v)[NR]) {
     const int nmem = run_nmem(hdr);
     const uint32_t flags = run_flags(hdr);
@@ -870,7 +1013,9 @@ __device__ __forceinline__ void diag_run(const CFOp *hp, const FOp &hdr, uint64_
         case kMemQ: q = run_q(m, base, q); break;
         case kMemR: break;
         case kMemS: sm = run_s(m, base, sm); break;
-        default: diag_gate<R, NR>(m, base, (base & m.cm_rest) == m.cm_rest, v); break;
+        default:
+            if constexpr (!Lean) diag_gate<R, NR>(m, base, (base & m.cm_rest) == m.cm_rest, v);
+            break;
         }
         m = mn;
     }
@@ -896,7 +1041,7 @@ __device__ __forceinline__ void acc_add(const AccCols &c, int slot, double v) {
 
 // Adjoint: the run applied to lambda (a) and the observable state (b); the
 // overlap of every trainable member is added to its slot of acc.
-template <class R, int NR, int RB>
+template <class R, int NR, int RB, bool Lean = false>
 __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, uint64_t base, cx<R> (&a)[NR],
                                              cx<R> (&b)[NR], const AccCols &acc) {
     const int nmem = run_nmem(hdr);
@@ -955,8 +1100,8 @@ __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, uin
             } else {
                 bool lane_ok = true;
                 if (m.cm_rest) lane_ok = (base & m.cm_rest) == m.cm_rest;
-                if (oc == kOvF) {
-                    val = diag_dot<R, NR>(m, diag_gen(m), base, lane_ok, a, b);
+                if (!Lean && oc == kOvF) {
+                    if constexpr (!Lean) val = diag_dot<R, NR>(m, diag_gen(m), base, lane_ok, a, b);
                 } else {
                     const double ge = m.m[2].re, go = m.m[3].re;
                     const bool lpar = __popcll(base & m.pm_rest) & 1;
@@ -980,9 +1125,11 @@ __device__ __forceinline__ void diag_run_adj(const CFOp *hp, const FOp &hdr, uin
         case kMemR: break;
         case kMemS: sm = run_s(m, base, sm); break;
         default: {
-            const bool lane_ok = (base & m.cm_rest) == m.cm_rest;
-            diag_gate<R, NR>(m, base, lane_ok, a);
-            diag_gate<R, NR>(m, base, lane_ok, b);
+            if constexpr (!Lean) {
+                const bool lane_ok = (base & m.cm_rest) == m.cm_rest;
+                diag_gate<R, NR>(m, base, lane_ok, a);
+                diag_gate<R, NR>(m, base, lane_ok, b);
+            }
             break;
         }
         }

@@ -128,6 +128,7 @@ def lib():
         L.lgh_stream_synchronize.argtypes = [vp]
         L.lgh_set_tile_bits.argtypes = [C.c_int, C.c_int]
         L.lgh_set_pass_waves.argtypes = [C.c_int64, C.c_int64]
+        L.lgh_set_lean_passes.argtypes = [C.c_int]
         L.lgh_autotune_fused.argtypes = [C.c_int, C.c_int32, C.c_int, vp, C.c_int64, C.POINTER(TuneResult)]
         L.lgh_fusion_plan.argtypes = [C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int] + [C.POINTER(C.c_int32)] * 4
         L.lgh_profile_enable.argtypes = [C.c_int]
@@ -506,6 +507,11 @@ def adjoint_jacobian_zwords(psi: State, zwires, ops, trainable, apply_operations
 # ---------------------------------------------------------------- launch timing
 
 PROFILE_KINDS = {"fused_pass": 0, "fused_adjoint": 1, "batch": 2}
+
+
+def set_lean_passes(enabled: bool):
+    """Run passes of lean gate forms in the lean kernels (default on)."""
+    check(lib().lgh_set_lean_passes(int(bool(enabled))))
 
 
 def profile_enable(enabled: bool = True):

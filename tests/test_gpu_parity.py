@@ -805,3 +805,34 @@ def test_adjoint_zwords_match_explicit_observable_states(dtype):
     bras = [po.apply_ops(n, st, [("PauliZ", [w], [], False) for w in ws]) for ws in zw]
     jref = po.adjoint_jacobian(n, st, bras, circ, tp)
     (close128 if dtype == np.complex128 else close64)(got, jref)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_lean_passes_match_general_kernels(dtype):
+    """Passes made only of uncontrolled real rotations and diagonal runs run
+    in the lean kernels (lgh_set_lean_passes, on by default).  Forward state
+    and adjoint Jacobian of the RY/RZ + CZ brick family (a lean circuit) and
+    of a circuit with controlled gates (general kernels) match the oracle
+    with the lean kernels on and off, and the two paths agree."""
+    from pennylane_lightning_gpu.workloads import ry_rz_cz_brick
+
+    close = close128 if dtype == np.complex128 else close64
+    rng = np.random.default_rng(5150)
+    for n, ops in ((14, ry_rz_cz_brick(14, 4, seed=77)), (13, controlled_fusable_circuit(13, rng, 150))):
+        psi = po.apply_ops(n, po.zero_state(n), ops)
+        bras = obs_states(n, psi, [[("PauliZ", [0]), ("PauliZ", [n - 1])]])
+        npar = sum(1 for o in ops if o[2])
+        tp = list(range(npar))
+        ref = po.adjoint_jacobian(n, psi, bras, ops, tp)
+        got = {}
+        try:
+            for lean in (True, False):
+                N.set_lean_passes(lean)
+                st = N.State(n, dtype).reset().apply(ops)
+                close(st.download(), psi)
+                jac = N.adjoint_jacobian(st, [N.State(n, dtype).upload(b.astype(dtype)) for b in bras], ops, tp)
+                close(jac, ref)
+                got[lean] = np.asarray(jac)
+        finally:
+            N.set_lean_passes(True)
+        close(got[True], got[False])
