@@ -138,9 +138,9 @@ lgh_status lgh_set_tile_bits(int forward, int adjoint);
  * counterpart in the reference; a tuning knob used by tests. */
 lgh_status lgh_set_pass_waves(int64_t forward, int64_t adjoint);
 /* Lean fused passes (on by default): a pass whose dense gates are all
- * uncontrolled real 2x2 matrices and whose diagonal gates all sit in
- * diagonal runs without general members runs in kernels compiled for just
- * those forms.  Results agree with the general kernels within rounding.  No
+ * uncontrolled 2x2 matrices (trainable ones with an X or Y generator) or
+ * controlled X gates, and whose diagonal gates all sit in diagonal runs
+ * without general members, runs in kernels compiled for just those forms.  Results agree with the general kernels within rounding.  No
  * counterpart in the reference; a switch for comparisons and tests. */
 lgh_status lgh_set_lean_passes(int enabled);
 /* Autotuner of the fused passes (no counterpart in the reference): times
@@ -179,6 +179,12 @@ lgh_status lgh_autotune_clear(void);
  * which applies one kernel per gate; for inspecting fusion and for tests. */
 lgh_status lgh_fusion_plan(int32_t num_qubits, const lgh_op *ops, int64_t num_ops, int32_t adjoint, lgh_dtype dtype,
                            int32_t *passes, int32_t *standalone, int32_t *runs, int32_t *run_members);
+/* Of the fused passes lgh_fusion_plan describes, how many run in the lean
+ * kernels (lgh_set_lean_passes): passes of uncontrolled 2x2 gates,
+ * controlled X gates and diagonal runs only.  Host only; no counterpart in
+ * the reference. */
+lgh_status lgh_fusion_lean_passes(int32_t num_qubits, const lgh_op *ops, int64_t num_ops, int32_t adjoint,
+                                  lgh_dtype dtype, int32_t *lean_passes);
 /* Launch timing of the hot kernels with HIP events recorded on their
  * launch stream (off by default; no counterpart in the reference).  Enabling
  * or disabling clears what was recorded.  kind 0: fused forward passes,

@@ -370,8 +370,9 @@ using namespace lgh;
 namespace {
 
 // Forward pass: every wavefront processes tiles [w*chunk, (w+1)*chunk).
-// Lean: the pass holds only the gate forms of a lean pass (lean_pass).
-template <class R, int RB, bool Lean = false>
+// Lv 1, 2: the pass holds only the gate forms of that lean level
+// (lean_level); 0: any fusable gate.
+template <class R, int RB, int Lv = 0>
 __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, uint64_t ntiles, uint64_t chunk,
                                                        const PassDesc *__restrict__ pd,
                                                        const FOp *__restrict__ ops) {
@@ -400,15 +401,15 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
         wait_op(cur);
         for (int o = 0; o < nops;) {
             if (f_kind(cur) == kDiagRun) {
-                diag_run<R, NR, Lean>(cops + o, cur, base, v);
+                diag_run<R, NR, (Lv > 0)>(cops + o, cur, base, v);
                 o += 1 + run_ndata(cur) + run_nmem(cur);
                 cur = load_op(cops + (o < nops ? o : 0));
                 wait_op(cur);
                 continue;
             }
             const FOp nxt = load_op(cops + (o + 1 < nops ? o + 1 : o)); // prefetch the next descriptor
-            if constexpr (Lean)
-                apply_lean<R, NR>(cur, lane, v);
+            if constexpr (Lv > 0)
+                apply_lean<R, NR, Lv>(cur, lane, base, v);
             else
                 apply_fop<R, NR>(cur, lane, base, v);
             cur = nxt;
@@ -434,7 +435,7 @@ __global__ __launch_bounds__(kBlock) void k_fused_pass(cx<R> *__restrict__ sv, u
 // (-1)^popcount(index & bra_zmask) per amplitude), then written like any
 // pass's output: the first reverse pass of a sweep with a Z-word observable
 // needs no separate pass that forms it.
-template <class R, int RB, bool Lean>
+template <class R, int RB, int Lv>
 __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                const FOp *__restrict__ ops, const FGen *__restrict__ gens,
@@ -486,7 +487,7 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
         wait_op(cur);
         for (int o = 0; o < nops;) {
             if (f_kind(cur) == kDiagRun) {
-                diag_run_adj<R, NR, RB, Lean>(cops + o, cur, base, a, b, cols);
+                diag_run_adj<R, NR, RB, (Lv > 0)>(cops + o, cur, base, a, b, cols);
                 o += 1 + run_ndata(cur) + run_nmem(cur);
                 cur = load_op(cops + (o < nops ? o : 0));
                 wait_op(cur);
@@ -496,11 +497,28 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
             const int slot = f_slot(cur);
             // overlaps go to the lane pair's LDS column (no contention; the
             // additions keep program order)
-            if constexpr (Lean) {
-                if (slot >= 0)
-                    acc_add(cols, slot, adj_plain_lean<R, NR, true>(cur, lane, a, b));
-                else
-                    adj_plain_lean<R, NR, false>(cur, lane, a, b);
+            if constexpr (Lv > 0) {
+                if (f_kind(cur) == kDiag) {
+                    if (Lv == 1 || !f_dsym(cur)) { // the pass's closing scale
+                        scale_lean<R, NR>(cur, a);
+                        scale_lean<R, NR>(cur, b);
+                    } else if constexpr (Lv == 2) {
+                        if (slot >= 0)
+                            acc_add(cols, slot, adj_diag_step<R, NR, true>(cur, cur.m[2].re, cur.m[3].re, base, a, b));
+                        else
+                            adj_diag_step<R, NR, false>(cur, 0.0, 0.0, base, a, b);
+                    }
+                } else if (Lv == 2 && !f_plain(cur)) { // a controlled X, not trainable
+                    if constexpr (Lv == 2) {
+                        const M2<R> m = m2<R>(cur);
+                        cx_lean<R, NR>(cur, m, lane, base, a);
+                        cx_lean<R, NR>(cur, m, lane, base, b);
+                    }
+                } else if (slot >= 0) {
+                    acc_add(cols, slot, adj_plain_lean<R, NR, true, Lv>(cur, lane, a, b));
+                } else {
+                    adj_plain_lean<R, NR, false, Lv>(cur, lane, a, b);
+                }
             } else if (f_kind(cur) != kDiag && f_plain(cur)) {
                 if (slot >= 0) {
                     // X and Y generators need no generator record
@@ -544,22 +562,22 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
 
 // Three register bits fit three wavefronts per SIMD (168 VGPRs, 48 KiB LDS
 // per workgroup); four register bits need all registers of one.
-// Lean: the pass holds only the gate forms of a lean pass (lean_pass).
-template <class R, bool Lean = false>
+// Lv: lean level of the pass (0: any fusable gate).
+template <class R, int Lv = 0>
 __global__ __launch_bounds__(kBlock, 3) void k_fused_adj3(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                        uint64_t ntiles, uint64_t chunk,
                                                        const PassDesc *__restrict__ pd, const FOp *__restrict__ ops,
                                                        const FGen *__restrict__ gens, double *__restrict__ partials,
                                                        uint64_t bra_zmask, int bra_from_lam) {
-    fused_adj_body<R, 3, Lean>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
+    fused_adj_body<R, 3, Lv>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
 }
-template <class R, bool Lean = false>
+template <class R, int Lv = 0>
 __global__ __launch_bounds__(kBlock) void k_fused_adj4(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                     uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                     const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                     double *__restrict__ partials, uint64_t bra_zmask,
                                                     int bra_from_lam) {
-    fused_adj_body<R, 4, Lean>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
+    fused_adj_body<R, 4, Lv>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
 }
 
 // slot sums -> jac[col[s]] * factor[s]
@@ -1040,9 +1058,9 @@ void run_forward(lgh_sv *sv, const std::vector<KOp> &kops, const std::vector<Pas
         std::memcpy(blob.data() + pb, b.ops.data(), b.ops.size() * sizeof(FOp));
         unsigned char *d = static_cast<unsigned char *>(ctx->upload(blob.data(), blob.size()));
         ProfScope prof(kProfFusedPass, ctx->stream);
-        const bool lean = g_lean.load() && lean_pass(b.ops, false);
-        auto kern = rb == 4 ? (lean ? k_fused_pass<R, 4, true> : k_fused_pass<R, 4>)
-                            : (lean ? k_fused_pass<R, 3, true> : k_fused_pass<R, 3>);
+        const int lv = g_lean.load() ? lean_level(b.ops, false) : 0;
+        auto kern = rb == 4 ? (lv == 1 ? k_fused_pass<R, 4, 1> : lv == 2 ? k_fused_pass<R, 4, 2> : k_fused_pass<R, 4>)
+                            : (lv == 1 ? k_fused_pass<R, 3, 1> : lv == 2 ? k_fused_pass<R, 3, 2> : k_fused_pass<R, 3>);
         hipLaunchKernelGGL(kern, dim3(geo.blocks), dim3(kBlock), 0, ctx->stream, static_cast<cx<R> *>(sv->data),
                            geo.ntiles, geo.chunk, reinterpret_cast<const PassDesc *>(d),
                            reinterpret_cast<const FOp *>(d + pb));
@@ -1094,6 +1112,7 @@ FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, 
         const bool single = dtype == LGH_C64;
         const Built b = adjoint ? build_pass(p, kops, n, true, &gens, &slot, rb, true, true, single)
                                 : build_pass(p, kops, n, false, nullptr, nullptr, rb, true, true, single);
+        plan.lean += lean_level(b.ops, adjoint) > 0 ? 1 : 0;
         for (size_t j = 0; j < b.ops.size();) {
             const FOp &f = b.ops[j];
             if ((f.meta & 15u) != kDiagRun) {
@@ -1164,9 +1183,9 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
         unsigned char *d = static_cast<unsigned char *>(ctx->upload(blob.data(), blob.size()));
         {
             ProfScope prof(kProfFusedAdj, ctx->stream);
-            const bool lean = g_lean.load() && lean_pass(b.ops, true);
-            auto kern = rb == 4 ? (lean ? k_fused_adj4<R, true> : k_fused_adj4<R>)
-                                : (lean ? k_fused_adj3<R, true> : k_fused_adj3<R>);
+            const int lv = g_lean.load() ? lean_level(b.ops, true) : 0;
+            auto kern = rb == 4 ? (lv == 1 ? k_fused_adj4<R, 1> : lv == 2 ? k_fused_adj4<R, 2> : k_fused_adj4<R>)
+                                : (lv == 1 ? k_fused_adj3<R, 1> : lv == 2 ? k_fused_adj3<R, 2> : k_fused_adj3<R>);
             const cx<R> *lam_in = static_cast<const cx<R> *>(src ? src : lam->data);
             src = nullptr;
             const uint64_t zmask = bra_pending ? bra_z->zmask : 0ull;
@@ -1216,6 +1235,17 @@ extern "C" lgh_status lgh_set_tile_bits(int forward, int adjoint) {
 
 extern "C" lgh_status lgh_set_pass_waves(int64_t forward, int64_t adjoint) {
     return guard([&] { set_pass_waves(forward, adjoint); });
+}
+
+extern "C" lgh_status lgh_fusion_lean_passes(int32_t num_qubits, const lgh_op *ops, int64_t num_ops, int32_t adjoint,
+                                             lgh_dtype dtype, int32_t *lean_passes) {
+    int32_t p = 0, s = 0, r = 0, m = 0;
+    const lgh_status st = lgh_fusion_plan(num_qubits, ops, num_ops, adjoint, dtype, &p, &s, &r, &m);
+    if (st != LGH_OK) return st;
+    return guard([&] {
+        LGH_REQUIRE(lean_passes, "null output");
+        *lean_passes = plan_passes(ops, num_ops, num_qubits, adjoint != 0, dtype).lean;
+    });
 }
 
 extern "C" lgh_status lgh_set_lean_passes(int enabled) {

@@ -49,7 +49,7 @@ std::shared_ptr<const std::vector<Pass>> schedule_passes_cached(const std::vecto
 // Host-side plan of the fused passes a gate sequence gets (no device use):
 // forward passes, or adjoint passes with every parametric gate trainable.
 struct FusionPlan {
-    int32_t passes = 0, standalone = 0, runs = 0, run_members = 0;
+    int32_t passes = 0, standalone = 0, runs = 0, run_members = 0, lean = 0;
 };
 FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, lgh_dtype dtype);
 

@@ -671,40 +671,49 @@ __device__ __forceinline__ double adj_plain_dense(const FOp &op, const FGen *g, 
     }
 }
 
-// Adjoint: a lean pass's dense gate (uncontrolled, real: rotation form 1 or
-// 2 or a plain real matrix; trainable ones with an X or Y generator).
-template <class R, int NR, int B, bool Gen, int Rot>
+// Adjoint: a lean pass's dense gate (uncontrolled: rotation form 1 or 2, a
+// plain real matrix (F = 0) or a complex one (F = 3); trainable ones with an
+// X or Y generator).
+template <class R, int NR, int B, bool Gen, int F>
 __device__ __forceinline__ double adj_reg_lean(const M2<R> &m, uint32_t gk, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
     const FGen g{};
+    constexpr bool Real = F != 3;
+    constexpr int Rot = F == 3 ? 0 : F;
     if constexpr (Gen) {
-        if (gk == kGenY) return adj_reg_plain<R, NR, B, true, true, Rot, kGenY>(m, gk, &g, a, b);
-        return adj_reg_plain<R, NR, B, true, true, Rot, kGenX>(m, gk, &g, a, b);
+        if (gk == kGenY) return adj_reg_plain<R, NR, B, Real, true, Rot, kGenY>(m, gk, &g, a, b);
+        return adj_reg_plain<R, NR, B, Real, true, Rot, kGenX>(m, gk, &g, a, b);
     }
-    return adj_reg_plain<R, NR, B, true, false, Rot>(m, gk, &g, a, b);
+    return adj_reg_plain<R, NR, B, Real, false, Rot>(m, gk, &g, a, b);
 }
-template <class R, int NR, int L, bool Gen, int Rot>
+template <class R, int NR, int L, bool Gen, int F>
 __device__ __forceinline__ double adj_lane_lean(const M2<R> &m, uint32_t gk, int lane, cx<R> (&a)[NR],
                                                 cx<R> (&b)[NR]) {
     const FGen g{};
+    constexpr bool Real = F != 3;
+    constexpr int Rot = F == 3 ? 0 : F;
     if constexpr (L >= 4 && Gen) {
-        if (gk == kGenY) return adj_lane_plain<R, NR, L, true, true, Rot, kGenY>(m, gk, &g, lane, a, b);
-        return adj_lane_plain<R, NR, L, true, true, Rot, kGenX>(m, gk, &g, lane, a, b);
+        if (gk == kGenY) return adj_lane_plain<R, NR, L, Real, true, Rot, kGenY>(m, gk, &g, lane, a, b);
+        return adj_lane_plain<R, NR, L, Real, true, Rot, kGenX>(m, gk, &g, lane, a, b);
     }
     // lane bits 0-3 take form 2 through the real path
-    return adj_lane_plain<R, NR, L, true, Gen, (L < 4 && Rot == 2) ? 0 : Rot>(m, gk, &g, lane, a, b);
+    return adj_lane_plain<R, NR, L, Real, Gen, (L < 4 && Rot == 2) ? 0 : Rot>(m, gk, &g, lane, a, b);
 }
-template <class R, int NR, bool Gen>
+template <class R, int NR, bool Gen, int Lv>
 __device__ __forceinline__ double adj_plain_lean(const FOp &op, int lane, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
     const M2<R> m = m2<R>(op);
     const uint32_t gk = f_gkind(op);
     const int rot = f_rot(op), loc = f_loc(op);
+    const int form = rot ? rot : ((Lv == 1 || f_real(op)) ? 0 : 3);
 #define LGH_REG3(B)                                                                                                    \
-    (rot == 1 ? adj_reg_lean<R, NR, B, Gen, 1>(m, gk, a, b)                                                          \
-              : rot == 2 ? adj_reg_lean<R, NR, B, Gen, 2>(m, gk, a, b) : adj_reg_lean<R, NR, B, Gen, 0>(m, gk, a, b))
+    (form == 1   ? adj_reg_lean<R, NR, B, Gen, 1>(m, gk, a, b)                                                       \
+     : form == 2 ? adj_reg_lean<R, NR, B, Gen, 2>(m, gk, a, b)                                                       \
+     : form == 0 ? adj_reg_lean<R, NR, B, Gen, 0>(m, gk, a, b)                                                       \
+                 : adj_reg_lean<R, NR, B, Gen, Lv == 2 ? 3 : 0>(m, gk, a, b))
 #define LGH_LANE3(L)                                                                                                   \
-    (rot == 1 ? adj_lane_lean<R, NR, L, Gen, 1>(m, gk, lane, a, b)                                                   \
-              : rot == 2 ? adj_lane_lean<R, NR, L, Gen, 2>(m, gk, lane, a, b)                                        \
-                         : adj_lane_lean<R, NR, L, Gen, 0>(m, gk, lane, a, b))
+    (form == 1   ? adj_lane_lean<R, NR, L, Gen, 1>(m, gk, lane, a, b)                                                \
+     : form == 2 ? adj_lane_lean<R, NR, L, Gen, 2>(m, gk, lane, a, b)                                                \
+     : form == 0 ? adj_lane_lean<R, NR, L, Gen, 0>(m, gk, lane, a, b)                                                \
+                 : adj_lane_lean<R, NR, L, Gen, Lv == 2 ? 3 : 0>(m, gk, lane, a, b))
     if (f_kind(op) == kRegDense) {
         switch (loc) {
         case 0: return LGH_REG3(0);
@@ -841,27 +850,50 @@ __device__ __forceinline__ FGen diag_gen(const FOp &op) {
 }
 
 // --- lean passes -------------------------------------------------------------
-// A pass whose dense gates are all uncontrolled real 2x2 matrices (the two
-// rotation forms or a plain real matrix) and whose diagonal gates are all
-// members of diagonal runs without general members runs in kernels compiled
-// with only those cases.  With the controlled and complex cases compiled in,
+// A pass whose dense gates are all uncontrolled 2x2 matrices (the two
+// rotation forms, plain real or complex matrices; trainable ones with an X
+// or Y generator) or controlled X gates (not trainable), and whose diagonal gates are all members of diagonal runs
+// without general members (or a pass's closing real scale), runs in kernels
+// compiled with only those cases.  With the controlled and complex cases compiled in,
 // the compiler keeps the loop-carried tile in one set of registers and each
 // gate's results in another, and copies the whole tile back after every
 // gate; without them the gates update the tile where it lies.
-// host test: is a built op allowed in a lean pass
-inline bool lean_op(const FOp &f, bool adjoint) {
+// a real factor on every amplitude (the closing scale gate of a pass
+// without diagonal runs)
+inline bool pure_scale(const FOp &f) {
+    return (f.meta & 15u) == kDiag && int((f.meta >> 4) & 15u) == kDiagNoReg && f.pm_rest == 0 &&
+           f.cm_rest == 0 && (f.rbits & 0xffffu) == 0xffffu && f.m[0].im == 0.0 && f.m[1].re == f.m[0].re &&
+           f.m[1].im == 0.0 && int((f.meta >> 8) & 255u) == 0;
+}
+// Lean level of a built op: 1 for the forms of level-1 kernels (uncontrolled
+// real 2x2 gates, diagonal runs, a pass's closing real scale), 2 for the
+// further forms of level-2 kernels (uncontrolled complex 2x2 gates,
+// controlled X gates, phases symmetric under conjugation on at most one
+// register bit), 0 for anything else.  Trainable dense gates need an X or Y
+// generator, trainable phases a diagonal one.  Each level is a kernel of its
+// own: the level-2 cases cost the level-1 passes a little.
+inline int lean_level_op(const FOp &f, bool adjoint) {
     const uint32_t kind = f.meta & 15u;
-    if (kind == kDiagRun) return true; // members are checked separately
-    if (kind != kRegDense && kind != kLaneDense) return false;
-    if ((f.rbits & 0xffffu) != 0xffffu || f.cm_rest != 0) return false; // controls
+    const bool trainable = adjoint && int((f.meta >> 8) & 255u) > 0;
+    if (kind == kDiagRun) return 1; // members are checked separately
+    if (kind == kDiag) {
+        if (pure_scale(f)) return 1;
+        const int loc = int((f.meta >> 4) & 15u);
+        const bool dsym = (f.meta >> 22) & 1u, plain = (f.rbits & 0xffffu) == 0xffffu && f.cm_rest == 0;
+        return dsym && plain && (loc < 4 || loc == kDiagNoReg) && (!trainable || ((f.meta >> 16) & 15u) == kGenDiag)
+                   ? 2
+                   : 0;
+    }
+    if (kind != kRegDense && kind != kLaneDense) return 0;
+    if ((f.rbits & 0xffffu) != 0xffffu || f.cm_rest != 0) // controls: only a controlled X
+        return ((f.meta >> 23) & 1u) && !trainable ? 2 : 0;
+    if (trainable) {
+        const uint32_t gk = (f.meta >> 16) & 15u;
+        if (gk != kGenX && gk != kGenY) return 0;
+    }
     const int rot = int((f.meta >> 28) & 3u);
     const bool real = (f.meta >> 21) & 1u;
-    if (rot == 0 && !real) return false;
-    if (adjoint && int((f.meta >> 8) & 255u) > 0) { // trainable: X or Y generator
-        const uint32_t gk = (f.meta >> 16) & 15u;
-        if (gk != kGenX && gk != kGenY) return false;
-    }
-    return true;
+    return rot != 0 || real ? 1 : 2;
 }
 inline bool lean_member(const FOp &m, bool adjoint) {
     const uint32_t mc = (m.meta >> 24) & 3u;
@@ -869,35 +901,95 @@ inline bool lean_member(const FOp &m, bool adjoint) {
     if (adjoint && int((m.meta >> 8) & 255u) > 0 && ((m.meta >> 26) & 3u) == kOvF) return false;
     return true;
 }
-inline bool lean_pass(const std::vector<FOp> &ops, bool adjoint) {
+// the lowest lean level whose kernels cover every op of a pass (0: none)
+inline int lean_level(const std::vector<FOp> &ops, bool adjoint) {
+    int level = 1;
     for (size_t j = 0; j < ops.size();) {
         const FOp &f = ops[j];
-        if (!lean_op(f, adjoint)) return false;
+        const int l = lean_level_op(f, adjoint);
+        if (l == 0) return 0;
+        level = l > level ? l : level;
         if ((f.meta & 15u) == kDiagRun) {
             const size_t nm = size_t(f.pad & 0xffffu), nd = size_t((f.pad >> 16) & 0xffu);
             for (size_t q = 0; q < nm; ++q)
-                if (!lean_member(ops[j + 1 + nd + q], adjoint)) return false;
+                if (!lean_member(ops[j + 1 + nd + q], adjoint)) return 0;
             j += 1 + nd + nm;
         } else {
             ++j;
         }
     }
-    return true;
+    return level;
+}
+
+// a controlled X (CNOT, Toffoli): the pair swaps where the controls hold
+template <class R, int NR>
+__device__ __forceinline__ void cx_lean(const FOp &op, const M2<R> &m, int lane, uint64_t base, cx<R> (&v)[NR]) {
+    const bool lane_ok = (base & op.cm_rest) == op.cm_rest;
+    const int loc = f_loc(op);
+    if (f_kind(op) == kRegDense) {
+        switch (loc) {
+        case 0: reg_masked<R, NR, 0, false, true>(op, m, lane_ok, v); break;
+        case 1: reg_masked<R, NR, 1, false, true>(op, m, lane_ok, v); break;
+        case 2: reg_masked<R, NR, 2, false, true>(op, m, lane_ok, v); break;
+        default:
+            if constexpr (NR > 8) reg_masked<R, NR, 3, false, true>(op, m, lane_ok, v);
+            break;
+        }
+        return;
+    }
+    switch (loc) {
+    case 0: lane_masked<R, NR, 0, false, true>(op, m, lane, lane_ok, v); break;
+    case 1: lane_masked<R, NR, 1, false, true>(op, m, lane, lane_ok, v); break;
+    case 2: lane_masked<R, NR, 2, false, true>(op, m, lane, lane_ok, v); break;
+    case 3: lane_masked<R, NR, 3, false, true>(op, m, lane, lane_ok, v); break;
+    case 4: lane_masked<R, NR, 4, false, true>(op, m, lane, lane_ok, v); break;
+    default: lane_masked<R, NR, 5, false, true>(op, m, lane, lane_ok, v); break;
+    }
 }
 
 // forward: a lean pass's dense gate
-template <class R, int NR>
-__device__ __forceinline__ void apply_lean(const FOp &op, int lane, cx<R> (&v)[NR]) {
+template <class R, int NR> __device__ __forceinline__ void scale_lean(const FOp &op, cx<R> (&v)[NR]) {
+    const R c = R(op.m[0].re);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) v[k] = rmul(c, v[k]);
+}
+template <class R, int NR, int Lv>
+__device__ __forceinline__ void apply_lean(const FOp &op, int lane, uint64_t base, cx<R> (&v)[NR]) {
+    if (f_kind(op) == kDiag) {
+        if (Lv == 1 || !f_dsym(op)) {
+            scale_lean<R, NR>(op, v); // a pure scale
+            return;
+        }
+        const bool lpar = __popcll(base & op.pm_rest) & 1;
+        switch (f_loc(op)) {
+        case 0: diag_sym<R, NR, 0>(op, lpar, v); return;
+        case 1: diag_sym<R, NR, 1>(op, lpar, v); return;
+        case 2: diag_sym<R, NR, 2>(op, lpar, v); return;
+        case 3:
+            if constexpr (NR > 8) diag_sym<R, NR, 3>(op, lpar, v);
+            return;
+        default: diag_sym<R, NR, -1>(op, lpar, v); return;
+        }
+    }
     const M2<R> m = m2<R>(op);
     const int loc = f_loc(op), rot = f_rot(op);
+    const bool real = Lv == 1 || f_real(op);
+    if constexpr (Lv == 2) {
+        if (!f_plain(op)) {
+            cx_lean<R, NR>(op, m, lane, base, v);
+            return;
+        }
+    }
     if (f_kind(op) == kRegDense) {
 #define LGH_REG_ARM(B)                                                                                                 \
     if (rot == 1)                                                                                                      \
         reg_plain<R, NR, B, true, 1>(m, v);                                                                            \
     else if (rot == 2)                                                                                                 \
         reg_plain<R, NR, B, true, 2>(m, v);                                                                            \
+    else if (real)                                                                                                     \
+        reg_plain<R, NR, B, true>(m, v);                                                                               \
     else                                                                                                               \
-        reg_plain<R, NR, B, true>(m, v);
+        reg_plain<R, NR, B, false>(m, v);
         switch (loc) {
         case 0: LGH_REG_ARM(0) break;
         case 1: LGH_REG_ARM(1) break;
@@ -914,8 +1006,10 @@ __device__ __forceinline__ void apply_lean(const FOp &op, int lane, cx<R> (&v)[N
         lane_plain<R, NR, L, true, 1>(m, lane, v);                                                                     \
     else if (L >= 4 && rot == 2)                                                                                       \
         lane_plain<R, NR, L, true, 2>(m, lane, v);                                                                     \
+    else if (real)                                                                                                     \
+        lane_plain<R, NR, L, true>(m, lane, v);                                                                        \
     else                                                                                                               \
-        lane_plain<R, NR, L, true>(m, lane, v);
+        lane_plain<R, NR, L, false>(m, lane, v);
     switch (loc) {
     case 0: LGH_LANE_ARM(0) break;
     case 1: LGH_LANE_ARM(1) break;

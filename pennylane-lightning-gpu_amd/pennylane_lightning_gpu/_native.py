@@ -131,6 +131,8 @@ def lib():
         L.lgh_set_lean_passes.argtypes = [C.c_int]
         L.lgh_autotune_fused.argtypes = [C.c_int, C.c_int32, C.c_int, vp, C.c_int64, C.POINTER(TuneResult)]
         L.lgh_fusion_plan.argtypes = [C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int] + [C.POINTER(C.c_int32)] * 4
+        L.lgh_fusion_lean_passes.argtypes = [C.c_int32, C.c_void_p, C.c_int64, C.c_int32, C.c_int,
+                                             C.POINTER(C.c_int32)]
         L.lgh_profile_enable.argtypes = [C.c_int]
         L.lgh_profile_read.argtypes = [C.c_int, C.POINTER(C.c_int64), C.POINTER(C.c_double)]
         L.lgh_tmpl_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(Op), C.c_int32, vp, vp, vp, C.c_int32,
@@ -230,6 +232,15 @@ def fusion_plan(n, ops, adjoint=False, dtype=np.complex128):
     code = LGH_C128 if np.dtype(dtype) == np.complex128 else LGH_C64
     check(lib().lgh_fusion_plan(int(n), arr, len(ops), int(bool(adjoint)), code, *[C.byref(o) for o in out]))
     return dict(zip(("passes", "standalone", "runs", "run_members"), (o.value for o in out)))
+
+
+def fusion_lean_passes(n, ops, adjoint=False, dtype=np.complex128) -> int:
+    """How many of the fused passes of ``ops`` run in the lean kernels (host only)."""
+    arr, keep = make_ops(ops) if ops else (None, None)
+    out = C.c_int32()
+    code = LGH_C128 if np.dtype(dtype) == np.complex128 else LGH_C64
+    check(lib().lgh_fusion_lean_passes(int(n), arr, len(ops), int(bool(adjoint)), code, C.byref(out)))
+    return out.value
 
 
 def device_count() -> int:

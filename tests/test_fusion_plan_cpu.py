@@ -108,3 +108,22 @@ def test_strongly_entangling_plan(n, layers):
     assert fwd["standalone"] == 0 and adj["standalone"] == 0
     assert fwd["passes"] <= 3 * layers and adj["passes"] <= 6 * layers
     assert adj["run_members"] == 2 * n * layers
+
+
+def test_lean_pass_coverage():
+    """Passes of uncontrolled 2x2 gates, controlled X gates and diagonal runs
+    run in the lean kernels (host plan); a controlled rotation or a trainable
+    gate without an X/Y generator keeps its pass on the general kernels."""
+    from pennylane_lightning_gpu.workloads import ladder, layered_ring, ry_rz_cz_brick, strongly_entangling
+
+    for n, ops in ((28, ry_rz_cz_brick(28, 6)), (27, strongly_entangling(27, 2)), (12, layered_ring(12, 4)),
+                   (24, ladder(24, 8))):
+        for adjoint in (False, True):
+            assert N.fusion_lean_passes(n, ops, adjoint=adjoint) == plan(n, ops, adjoint)["passes"]
+    ops = ry_rz_cz_brick(14, 2) + [("CRX", [3, 9], [0.4], False)]
+    for adjoint in (False, True):
+        p = plan(14, ops, adjoint)["passes"]
+        assert N.fusion_lean_passes(14, ops, adjoint=adjoint) == p - 1
+    # a trainable PhaseShift (generator not X/Y) on a register bit is lean only forward
+    ops = ry_rz_cz_brick(12, 2) + [("PhaseShift", [5], [0.3], False)]
+    assert N.fusion_lean_passes(12, ops) == plan(12, ops)["passes"]
