@@ -137,7 +137,8 @@ lgh_status lgh_set_tile_bits(int forward, int adjoint);
  * amplitudes, so a small cap exercises the multi-tile path at any size.  No
  * counterpart in the reference; a tuning knob used by tests. */
 lgh_status lgh_set_pass_waves(int64_t forward, int64_t adjoint);
-/* Lean fused passes (on by default): a pass whose dense gates are all
+/* Lean fused passes (on by default; single-state passes and the batched
+ * kernel): a pass whose dense gates are all
  * uncontrolled 2x2 matrices (trainable ones with an X or Y generator) or
  * controlled X gates, and whose diagonal gates all sit in diagonal runs
  * without general members, runs in kernels compiled for just those forms.  Results agree with the general kernels within rounding.  No

@@ -209,7 +209,8 @@ __device__ __forceinline__ uint64_t tile_step(uint64_t g, uint64_t tmask) {
     return g;
 }
 
-template <class R>
+// Lv > 0: every gate of the pass has a form of that lean level (see lean_level)
+template <class R, int Lv>
 __device__ __forceinline__ void fwd_pass(const CTPass *tp, const CFOp *fops, cx<R> *psi, uint64_t ntiles, int w, int lane) {
     const PassGeo g = pass_geo(tp, lane);
     const int nops = tp->nops;
@@ -224,7 +225,10 @@ __device__ __forceinline__ void fwd_pass(const CTPass *tp, const CFOp *fops, cx<
             FOp cur = load_op(ops);
             for (int o = 0; o < nops; ++o) {
                 const FOp nxt = load_op(ops + (o + 1 < nops ? o + 1 : o));
-                apply_fop<R, NR>(cur, lane, base, v);
+                if constexpr (Lv > 0)
+                    apply_lean<R, NR, Lv>(cur, lane, base, v);
+                else
+                    apply_fop<R, NR>(cur, lane, base, v);
                 cur = nxt;
             }
         }
@@ -257,7 +261,7 @@ __device__ __forceinline__ double obs_pass(const CTPass *tp, const CFOp *fops, c
     return e;
 }
 
-template <class R>
+template <class R, int Lv>
 __device__ __forceinline__ void rev_pass(const CTPass *tp, const CFOp *fops, const CFGen *gens, cx<R> *lam, cx<R> *bra,
                          uint64_t ntiles, int w, int lane, double (*acc)[kBlock]) {
     const PassGeo g = pass_geo(tp, lane);
@@ -276,7 +280,26 @@ __device__ __forceinline__ void rev_pass(const CTPass *tp, const CFOp *fops, con
             for (int o = 0; o < nops; ++o) {
                 const FOp nxt = load_op(ops + (o + 1 < nops ? o + 1 : o));
                 const int slot = f_slot(cur);
-                if (f_kind(cur) != kDiag && f_plain(cur)) {
+                if constexpr (Lv > 0) {
+                    if (f_kind(cur) == kDiag) { // a phase on at most one register bit
+                        if (slot >= 0) {
+                            const CFGen *gp = gens + slot;
+                            acc[slot][tid] += adj_diag_step<R, NR, true>(cur, gp->g[0].re, gp->g[1].re, base, a, b);
+                        } else {
+                            adj_diag_step<R, NR, false>(cur, 0.0, 0.0, base, a, b);
+                        }
+                    } else if (Lv == 2 && !f_plain(cur)) { // a controlled X, not trainable
+                        if constexpr (Lv == 2) {
+                            const M2<R> m = m2<R>(cur);
+                            cx_lean<R, NR>(cur, m, lane, base, a);
+                            cx_lean<R, NR>(cur, m, lane, base, b);
+                        }
+                    } else if (slot >= 0) {
+                        acc[slot][tid] += adj_plain_lean<R, NR, true, Lv>(cur, lane, a, b);
+                    } else {
+                        adj_plain_lean<R, NR, false, Lv>(cur, lane, a, b);
+                    }
+                } else if (f_kind(cur) != kDiag && f_plain(cur)) {
                     if (slot >= 0) {
                         const FGen gg = load_gen(gens + slot);
                         acc[slot][tid] += adj_plain_dense<R, NR, true>(cur, &gg, lane, a, b);
@@ -309,7 +332,9 @@ __device__ __forceinline__ void rev_pass(const CTPass *tp, const CFOp *fops, con
 
 // (two wavefronts per SIMD: forcing three spills registers inside the gate
 // loop and halves throughput)
-template <class R>
+// Lv: lean level of every forward and reverse pass of the batch's templates
+// (0: any fusable gate); the observable pass always takes the general form.
+template <class R, int Lv = 0>
 __global__ __launch_bounds__(kBlock) void k_batch(const CircDev *__restrict__ circs, int ncirc,
                                                   uint32_t *__restrict__ counter, int grad) {
     __shared__ double acc[kMaxSlots][kBlock];
@@ -338,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const CircDev *__restrict__ ci
             for (int j = tid; j < nparams; j += kBlock) jac[j] = 0.0;
         block_fence();
         for (int p = 0; p < nfwd; ++p) {
-            fwd_pass<R>((const CTPass *)t->fwd + p, fops, psi, ntiles, w, lane);
+            fwd_pass<R, Lv>((const CTPass *)t->fwd + p, fops, psi, ntiles, w, lane);
             block_fence();
         }
         const double e = wave_sum(obs_pass<R>((const CTPass *)t->obs, fops, psi, bra, ntiles, w, lane));
@@ -356,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const CircDev *__restrict__ ci
                 const CTPass *tp = (const CTPass *)t->rev + p;
                 const int ns = tp->nslots, s0 = tp->slot_begin;
                 for (int s = 0; s < ns; ++s) acc[s][tid] = 0.0;
-                rev_pass<R>(tp, fops, gens + s0, psi, bra, ntiles, w, lane, acc);
+                rev_pass<R, Lv>(tp, fops, gens + s0, psi, bra, ntiles, w, lane, acc);
                 __threadfence();
                 __syncthreads();
                 for (int s = w; s < ns; s += kWaves) {
@@ -389,6 +414,7 @@ struct lgh_tmpl {
     std::vector<TSlot> slots;
     std::vector<Recipe> recipes;
     uint64_t cost = 0; // tiles x passes, for ordering a batch
+    int lean = 0;      // lean level of every forward and reverse pass (0: general kernels)
     // the circuit as given, for the serial form
     std::vector<lgh_op> ops;
     std::vector<std::vector<int32_t>> wires;
@@ -501,6 +527,7 @@ struct BatchLayout {
     uint64_t ns = 0;
     size_t b_circ = 0, b_par = 0, b_fop = 0, b_jac = 0, b_exp = 0, total = 0, sbytes = 0;
     std::vector<int32_t> order;
+    int lean = 0; // the kernel's lean level: the highest of the templates', 0 if any has none
     char *d = nullptr;
     void *states = nullptr;
 
@@ -514,6 +541,14 @@ struct BatchLayout {
         dev = tmpls[0]->device;
         ncirc = n;
         grad = with_grad;
+        lean = lean_enabled() ? 1 : 0;
+        for (int32_t i = 0; i < n && lean > 0; ++i) {
+            if (tmpls[i]->lean == 0) {
+                lean = 0;
+                break;
+            }
+            lean = std::max(lean, tmpls[i]->lean);
+        }
         p_off.resize(size_t(n));
         f_off.resize(size_t(n));
         j_off.resize(size_t(n));
@@ -583,13 +618,18 @@ unsigned batch_grid(int dev, int32_t ncirc, int per_cu) {
     return unsigned(std::min<int64_t>(ncirc, int64_t(per_cu) * int64_t(std::max(1, sm_count(dev)))));
 }
 
+typedef void (*BatchKernel)(const CircDev *, int, uint32_t *, int);
+template <class R> BatchKernel batch_kernel(int lean) {
+    return lean == 1 ? k_batch<R, 1> : lean == 2 ? k_batch<R, 2> : k_batch<R, 0>;
+}
+
 template <class R>
 void launch_batch(const BatchLayout &L, unsigned grid, hipStream_t stream) {
     hipLaunchKernelGGL(k_batch_prep, dim3(unsigned(L.ncirc)), dim3(kBlock), 0, stream,
                        reinterpret_cast<const CircDev *>(L.d), L.d_cnt());
     LGH_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_batch<R>, dim3(grid), dim3(kBlock), 0, stream, reinterpret_cast<const CircDev *>(L.d),
-                       int(L.ncirc), L.d_cnt(), int(L.grad));
+    hipLaunchKernelGGL(batch_kernel<R>(L.lean), dim3(grid), dim3(kBlock), 0, stream,
+                       reinterpret_cast<const CircDev *>(L.d), int(L.ncirc), L.d_cnt(), int(L.grad));
     LGH_LAUNCH_CHECK();
 }
 
@@ -681,8 +721,8 @@ struct lgh_batch_plan {
         if (err == hipSuccess) {
             hipLaunchKernelGGL(k_batch_prep, dim3(unsigned(L.ncirc)), dim3(kBlock), 0, stream,
                                reinterpret_cast<const CircDev *>(L.d), L.d_cnt());
-            hipLaunchKernelGGL(k_batch<R>, dim3(batch_grid(dev, L.ncirc, per_cu)), dim3(kBlock), 0, stream,
-                               reinterpret_cast<const CircDev *>(L.d), int(L.ncirc), L.d_cnt(), int(L.grad));
+            hipLaunchKernelGGL(batch_kernel<R>(L.lean), dim3(batch_grid(dev, L.ncirc, per_cu)), dim3(kBlock), 0,
+                               stream, reinterpret_cast<const CircDev *>(L.d), int(L.ncirc), L.d_cnt(), int(L.grad));
             err = hipGetLastError();
         }
         if (err == hipSuccess)
@@ -1019,6 +1059,32 @@ lgh_status lgh_tmpl_create(lgh_tmpl **out, int32_t num_qubits, const lgh_op *ops
         }
         t->nfops = nf;
         t->cost = (1ull << (nq - kTileBits)) * (t->fwd.size() + 2 * t->rev.size() + 2);
+        // lean level of the forward and reverse passes, for every parameter value: the
+        // prep kernel recomputes the real and symmetric-phase flags of parametrised gates
+        auto recipe_level = [](const Recipe &r, bool rev) -> int {
+            const FOp &f = r.base;
+            const uint32_t kind = f.meta & 15u;
+            const bool dense = kind == kRegDense || kind == kLaneDense;
+            const bool plain = (f.rbits & 0xffffu) == 0xffffu && f.cm_rest == 0;
+            const int loc = int((f.meta >> 4) & 15u);
+            switch (r.form) {
+            case kRY: return dense && plain ? 1 : 0;  // real for every angle; Y generator
+            case kRX: return dense && plain ? 2 : 0;  // X generator
+            case kRotDiag: // d_odd = conj(d_even) for every angle; diagonal generator
+                return kind == kDiag && plain && (loc < 4 || loc == kDiagNoReg) ? 2 : 0;
+            case kPhase: return 0;
+            default: return lean_level_op(f, rev); // fixed: as built, never trainable
+            }
+        };
+        t->lean = 1;
+        auto take = [&](const TPass &tp, bool rev) {
+            for (int32_t j = 0; j < tp.nops && t->lean > 0; ++j) {
+                const int l = recipe_level(t->recipes[size_t(tp.op_begin + j)], rev);
+                t->lean = l == 0 ? 0 : std::max(t->lean, l);
+            }
+        };
+        for (const TPass &tp : t->fwd) take(tp, false);
+        for (const TPass &tp : t->rev) take(tp, true);
         *out = t.release();
     });
 }

@@ -332,6 +332,8 @@ SchedCache &sched_cache() {
 constexpr size_t kSchedCacheMax = 512;
 } // namespace
 
+bool lean_enabled() { return g_lean.load(); }
+
 std::shared_ptr<const std::vector<Pass>> schedule_passes_cached(const std::vector<KOp> &ops, int n,
                                                                 const std::vector<int> &trainable_slot, int rb,
                                                                 int fixed_lanes, int max_slots) {

@@ -46,6 +46,8 @@ std::vector<Pass> schedule_passes(const std::vector<KOp> &ops, int n, const std:
 std::shared_ptr<const std::vector<Pass>> schedule_passes_cached(const std::vector<KOp> &ops, int n,
                                                                 const std::vector<int> &trainable_slot, int rb,
                                                                 int fixed_lanes, int max_slots);
+// lgh_set_lean_passes: passes of lean gate forms run in the lean kernels
+bool lean_enabled();
 // Host-side plan of the fused passes a gate sequence gets (no device use):
 // forward passes, or adjoint passes with every parametric gate trainable.
 struct FusionPlan {

@@ -233,3 +233,38 @@ def test_batch_c64_multi_circuit_matches_oracle():
         e, g = oracle_run(w, *ring_template(w), thetas[c], [("Z", 0)])
         close64(ev[c], e)
         close64(jac[c], g)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_batch_lean_kernel_matches_general_and_oracle(dtype):
+    """Batches whose templates hold only lean gate forms (the RX/RY/RZ +
+    CNOT ring) run in the lean batched kernel; a template with controlled
+    rotations (CRY) sends the whole batch to the general kernel.  Both, with
+    the lean kernels on and off, match the oracle."""
+    close = close128 if dtype == np.complex128 else close64
+    rng = np.random.default_rng(77)
+    ring = {n: N.Template(n, *ring_template(n), [("Z", 0)]) for n in (9, 12)}
+    ops = [("RX", [0], [0.0], False), ("CRY", [0, 5], [0.0], False), ("CNOT", [5, 2], [], False),
+           ("RZ", [2], [0.0], False), ("RY", [7], [0.0], False)]
+    mixed = N.Template(10, ops, [0, 1, -1, 2, 3], [("Z", 2)])
+    for batch in ([ring[9], ring[12], ring[9]], [ring[12], mixed, ring[9]]):
+        thetas = [rng.uniform(0, 2 * np.pi, t.num_params) for t in batch]
+        got = {}
+        try:
+            for lean in (True, False):
+                N.set_lean_passes(lean)
+                plan = N.BatchPlan(batch, grad=True, dtype=dtype)
+                got[lean] = plan.run(thetas)
+                plan.close()
+        finally:
+            N.set_lean_passes(True)
+        for lean in (True, False):
+            ev, jac = got[lean]
+            for c, t in enumerate(batch):
+                n = t.num_qubits
+                if t is mixed:
+                    e, g = oracle_run(n, ops, [0, 1, -1, 2, 3], thetas[c], [("Z", 2)])
+                else:
+                    e, g = oracle_run(n, *ring_template(n), thetas[c], [("Z", 0)])
+                close(ev[c], e)
+                close(jac[c], g)
