@@ -755,13 +755,20 @@ def main():
     fwd_avg = fwd_ms / max(1, fwd_n)
     # HBM bytes per launch from the committed PMC summary, used only if it
     # was measured on the current kernel sources
-    traffic, traffic_note = None, None
+    traffic, traffic_note, valu = None, None, None
     tfile = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if os.path.exists(tfile) and n == 28:
         with open(tfile) as f:
             entry = json.load(f).get(f"k_fused_adj<double>@{n}q", {})
         if entry.get("kernel_sources_sha16") == kernel_sources_sha16():
             traffic = entry.get("traffic_bytes")
+            if entry.get("valu_insts"):
+                # VALU issue: a wave64 instruction occupies a SIMD for 4 cycles; 1024 SIMDs
+                # at the 2.4 GHz peak engine clock
+                issue_s = entry["valu_insts"] * 4 / (1024 * 2.4e9)
+                valu = {"insts_per_launch": int(entry["valu_insts"]),
+                        "issue_ms_at_peak_clock": round(issue_s * 1e3, 4),
+                        "issue_frac": round(issue_s / (adj_avg * 1e-3), 3)}
         else:
             traffic_note = "PMC summary predates the current kernel sources: not used"
     kernel_stats = {
@@ -840,10 +847,12 @@ def main():
         "config": {"workload": f"adjoint Jacobian, {n} qubits c128, {layers} layers RY/RZ + CZ brick, "
                                f"{len(ops)} ops, {n_par} params, <Z0 Z{n - 1}>", "seed": args.seed,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_fused_adj<double> (fused adjoint pass: lambda + 1 observable state)",
+        "roofline": {"bound": "hbm", "kernel": "k_fused_adj3<double, 1> (fused adjoint pass, lean level 1: "
+                                               "lambda + 1 observable state)",
                      "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fused_gbs / HBM_PEAK_GBS, 3), "traffic": traffic, "traffic_note": traffic_note,
-                     "bytes_per_launch": int(fused_bytes), "ms_per_launch": round(adj_avg, 4)},
+                     "bytes_per_launch": int(fused_bytes), "ms_per_launch": round(adj_avg, 4),
+                     "valu_issue": valu},
         "cpu_baseline": cpu,
         "autotune": tune,
         "kernels": kernel_stats,
