@@ -295,9 +295,13 @@ __device__ __forceinline__ void rev_pass(const CTPass *tp, const CFOp *fops, con
                             cx_lean<R, NR>(cur, m, lane, base, b);
                         }
                     } else if (slot >= 0) {
-                        acc[slot][tid] += adj_plain_lean<R, NR, true, Lv>(cur, lane, a, b);
+                        // one add after the dispatch measured faster here than an add
+                        // in every branch
+                        double s = 0.0;
+                        adj_plain_lean<R, NR, true, Lv>(cur, lane, a, b, [&](double v) { s = v; });
+                        acc[slot][tid] += s;
                     } else {
-                        adj_plain_lean<R, NR, false, Lv>(cur, lane, a, b);
+                        adj_plain_lean<R, NR, false, Lv>(cur, lane, a, b, [](double) {});
                     }
                 } else if (f_kind(cur) != kDiag && f_plain(cur)) {
                     if (slot >= 0) {

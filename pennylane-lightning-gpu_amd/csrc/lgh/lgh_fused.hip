@@ -442,17 +442,17 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
                                                uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                double *__restrict__ partials, uint64_t bra_zmask, int bra_from_lam) {
-    static_assert(kAccCols * kAccLanes == kBlock, "one accumulator column per lane pair");
+    static_assert(kAccCols * kAccLanes == kBlock, "one accumulator column per lane quad");
     __shared__ double acc[kMaxSlotsSv][kAccCols];
-    __shared__ double sink[kAccCols];
+    __shared__ double sink[kBlock];
     const AccCols cols{acc, sink};
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const CFOp *cops = (const CFOp *)ops;
     const CFGen *cgens = (const CFGen *)gens;
     const int nslots = pd->nslots;
-    // each column is cleared by the lane pair that adds to it (both lanes
-    // store the same zero)
-    for (int s = 0; s < nslots; ++s) acc[s][threadIdx.x >> 1] = 0.0;
+    // each column is cleared by the lane quad that adds to it (all four
+    // lanes store the same zero)
+    for (int s = 0; s < nslots; ++s) acc[s][threadIdx.x >> 2] = 0.0;
     const uint64_t wave = xcd_block() * kWavesPerBlock + w;
     const uint64_t t0 = wave * chunk, t1 = min(t0 + chunk, ntiles);
     const int nops = pd->nops;
@@ -517,9 +517,9 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
                         cx_lean<R, NR>(cur, m, lane, base, b);
                     }
                 } else if (slot >= 0) {
-                    acc_add(cols, slot, adj_plain_lean<R, NR, true, Lv>(cur, lane, a, b));
+                    adj_plain_lean<R, NR, true, Lv>(cur, lane, a, b, [&](double v) { acc_add(cols, slot, v); });
                 } else {
-                    adj_plain_lean<R, NR, false, Lv>(cur, lane, a, b);
+                    adj_plain_lean<R, NR, false, Lv>(cur, lane, a, b, [](double) {});
                 }
             } else if (f_kind(cur) != kDiag && f_plain(cur)) {
                 if (slot >= 0) {
@@ -554,16 +554,18 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
         }
     }
     __syncthreads();
-    // slot s: wavefront w sums the 128 columns in a fixed order
+    // slot s: wavefront w sums the 64 columns in a fixed order
+    static_assert(kAccCols == 64, "one column per lane of the summing wavefront");
     for (int s = w; s < nslots; s += kWavesPerBlock) {
-        double v = acc[s][lane] + acc[s][lane + 64];
+        double v = acc[s][lane];
         v = wave_sum(v);
         if (lane == 0) partials[size_t(s) * gridDim.x + blockIdx.x] = v;
     }
 }
 
-// Three register bits fit three wavefronts per SIMD (168 VGPRs, 48 KiB LDS
-// per workgroup); four register bits need all registers of one.
+// Three register bits fit three wavefronts per SIMD in the general kernel
+// (at most 168 VGPRs) and four in the lean ones (115 VGPRs; 26 KiB of LDS per
+// workgroup); four register bits need all registers of one.
 // Lv: lean level of the pass (0: any fusable gate).
 template <class R, int Lv = 0>
 __global__ __launch_bounds__(kBlock, 3) void k_fused_adj3(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,

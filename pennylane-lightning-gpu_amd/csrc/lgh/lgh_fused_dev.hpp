@@ -23,13 +23,15 @@ constexpr int kLaneBits = 6;
 // Trainable gates per pass.  The adjoint kernels keep their overlap sums in
 // LDS, 48 KiB per workgroup of 256 lanes (three workgroups per CU): the
 // batched kernel one double per slot and lane (kMaxSlots), the single-state
-// kernel one per slot and pair of lanes (kMaxSlotsSv), the pair's two values
-// added with one lane exchange first.  A pass holds as many trainable gates as
-// it has slots, so twice the slots means fewer passes over the two states.
+// kernel one per slot and quad of lanes (kMaxSlotsSv), the quad's four values
+// added with two lane exchanges first.  A pass holds as many trainable gates
+// as it has slots, so twice the slots means fewer passes over the two states;
+// a quarter of a column per lane keeps the workgroup's LDS at 26 KiB, which
+// lets four workgroups (four waves per SIMD) share a CU.
 constexpr int kMaxSlots = 24;
-constexpr int kAccLanes = 2;                     // lanes per accumulator column (single-state kernel)
+constexpr int kAccLanes = 4;                     // lanes per accumulator column (single-state kernel)
 constexpr int kAccCols = 256 / kAccLanes;        // columns per workgroup
-constexpr int kMaxSlotsSv = kMaxSlots * kAccLanes;
+constexpr int kMaxSlotsSv = 2 * kMaxSlots;
 constexpr int kRBForward = 4;  // forward passes (one state)
 constexpr int kRBAdjoint = 3;  // adjoint passes (lambda and one observable state)
 
@@ -698,39 +700,44 @@ __device__ __forceinline__ double adj_lane_lean(const M2<R> &m, uint32_t gk, int
     // lane bits 0-3 take form 2 through the real path
     return adj_lane_plain<R, NR, L, Real, Gen, (L < 4 && Rot == 2) ? 0 : Rot>(m, gk, &g, lane, a, b);
 }
-template <class R, int NR, bool Gen, int Lv>
-__device__ __forceinline__ double adj_plain_lean(const FOp &op, int lane, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
+// `sink` takes a trainable gate's overlap inside the gate's own branch: with
+// one call after the dispatch, every branch's tile results met in a common
+// block, where the register allocator gave the tile a second register set
+// and copied it in and out (32-64 moves per trainable gate).
+template <class R, int NR, bool Gen, int Lv, class Sink>
+__device__ __forceinline__ void adj_plain_lean(const FOp &op, int lane, cx<R> (&a)[NR], cx<R> (&b)[NR],
+                                               const Sink &sink) {
     const M2<R> m = m2<R>(op);
     const uint32_t gk = f_gkind(op);
     const int rot = f_rot(op), loc = f_loc(op);
     const int form = rot ? rot : ((Lv == 1 || f_real(op)) ? 0 : 3);
 #define LGH_REG3(B)                                                                                                    \
-    (form == 1   ? adj_reg_lean<R, NR, B, Gen, 1>(m, gk, a, b)                                                       \
-     : form == 2 ? adj_reg_lean<R, NR, B, Gen, 2>(m, gk, a, b)                                                       \
-     : form == 0 ? adj_reg_lean<R, NR, B, Gen, 0>(m, gk, a, b)                                                       \
-                 : adj_reg_lean<R, NR, B, Gen, Lv == 2 ? 3 : 0>(m, gk, a, b))
+    (form == 1   ? sink(adj_reg_lean<R, NR, B, Gen, 1>(m, gk, a, b))                                                 \
+     : form == 2 ? sink(adj_reg_lean<R, NR, B, Gen, 2>(m, gk, a, b))                                                 \
+     : form == 0 ? sink(adj_reg_lean<R, NR, B, Gen, 0>(m, gk, a, b))                                                 \
+                 : sink(adj_reg_lean<R, NR, B, Gen, Lv == 2 ? 3 : 0>(m, gk, a, b)))
 #define LGH_LANE3(L)                                                                                                   \
-    (form == 1   ? adj_lane_lean<R, NR, L, Gen, 1>(m, gk, lane, a, b)                                                \
-     : form == 2 ? adj_lane_lean<R, NR, L, Gen, 2>(m, gk, lane, a, b)                                                \
-     : form == 0 ? adj_lane_lean<R, NR, L, Gen, 0>(m, gk, lane, a, b)                                                \
-                 : adj_lane_lean<R, NR, L, Gen, Lv == 2 ? 3 : 0>(m, gk, lane, a, b))
+    (form == 1   ? sink(adj_lane_lean<R, NR, L, Gen, 1>(m, gk, lane, a, b))                                          \
+     : form == 2 ? sink(adj_lane_lean<R, NR, L, Gen, 2>(m, gk, lane, a, b))                                          \
+     : form == 0 ? sink(adj_lane_lean<R, NR, L, Gen, 0>(m, gk, lane, a, b))                                          \
+                 : sink(adj_lane_lean<R, NR, L, Gen, Lv == 2 ? 3 : 0>(m, gk, lane, a, b)))
     if (f_kind(op) == kRegDense) {
         switch (loc) {
-        case 0: return LGH_REG3(0);
-        case 1: return LGH_REG3(1);
-        case 2: return LGH_REG3(2);
+        case 0: LGH_REG3(0); return;
+        case 1: LGH_REG3(1); return;
+        case 2: LGH_REG3(2); return;
         default:
-            if constexpr (NR > 8) return LGH_REG3(3);
-            return 0.0;
+            if constexpr (NR > 8) LGH_REG3(3);
+            return;
         }
     }
     switch (loc) {
-    case 0: return LGH_LANE3(0);
-    case 1: return LGH_LANE3(1);
-    case 2: return LGH_LANE3(2);
-    case 3: return LGH_LANE3(3);
-    case 4: return LGH_LANE3(4);
-    default: return LGH_LANE3(5);
+    case 0: LGH_LANE3(0); return;
+    case 1: LGH_LANE3(1); return;
+    case 2: LGH_LANE3(2); return;
+    case 3: LGH_LANE3(3); return;
+    case 4: LGH_LANE3(4); return;
+    default: LGH_LANE3(5); return;
     }
 #undef LGH_REG3
 #undef LGH_LANE3
@@ -1039,56 +1046,63 @@ template <class R> __device__ __forceinline__ cx<R> run_q(const FOp &m, uint64_t
 __device__ __forceinline__ uint32_t run_s(const FOp &m, uint64_t base, uint32_t sm) {
     return sm ^ (((base & m.cm_rest) == m.cm_rest) ? (m.rbits & 0xffffu) : 0u);
 }
-// factor of register K at the end of a run
-// HasQ: the run has per-lane members; without them q is 1 and the factor of
-// a register is its constant P[k] (with its sign)
-template <class R, int K, bool HasQ>
-__device__ __forceinline__ cx<R> run_factor(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm) {
+// The end of a run: register k's factor is the lane factor q (HasQ: the run
+// has per-lane members), times its constant P[k] (HasP: the data ops exist
+// only then), with its sign from bit k of sm (HasS).  The flags are uniform and
+// fixed per variant, so each variant is straight-line code that forms a factor
+// and applies it register by register (with per-register branches the factors
+// of all registers were formed first and held at once: 64 more VGPRs at four
+// register bits).  A run with signs only flips signs.
+template <class R, bool HasQ, bool HasP, bool HasS>
+__device__ __forceinline__ cx<R> run_factor(const CFOp *hp, int k, cx<R> q, uint32_t sm) {
     cx<R> f = HasQ ? q : cx<R>{R(1), R(0)};
-    if (flags & kRunHasP) { // the data ops exist only then
-        const cx<R> p = lo<R>(cx<double>{hp[1 + K / 4].m[K % 4].re, hp[1 + K / 4].m[K % 4].im});
-        if constexpr (HasQ)
-            f = cmul(f, p);
-        else
-            f = p;
+    if constexpr (HasP) {
+        const cx<R> p = lo<R>(cx<double>{hp[1 + k / 4].m[k % 4].re, hp[1 + k / 4].m[k % 4].im});
+        f = HasQ ? cmul(f, p) : p;
     }
-    if (flags & kRunHasS) {
-        const uint32_t fm = (sm << (31 - K)) & kSignBit;
+    if constexpr (HasS) {
+        const uint32_t fm = (sm << (31 - k)) & kSignBit;
         f = cx<R>{flip(f.re, fm), flip(f.im, fm)};
     }
     return f;
 }
-template <class R, int NR, bool HasQ, int K = 0>
-__device__ __forceinline__ void run_finish_q(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR]) {
-    if constexpr (K < NR) {
-        a[K] = cmul(a[K], run_factor<R, K, HasQ>(hp, flags, q, sm));
-        run_finish_q<R, NR, HasQ, K + 1>(hp, flags, q, sm, a);
+template <class R, int NR, int NS, bool HasQ, bool HasP, bool HasS>
+__device__ __forceinline__ void run_finish_t(const CFOp *hp, cx<R> q, uint32_t sm, cx<R> (&a)[NR], cx<R> (&b)[NR]) {
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+        if constexpr (!HasQ && !HasP) {
+            const uint32_t fm = (sm << (31 - k)) & kSignBit;
+            a[k] = cx<R>{flip(a[k].re, fm), flip(a[k].im, fm)};
+            if constexpr (NS == 2) b[k] = cx<R>{flip(b[k].re, fm), flip(b[k].im, fm)};
+        } else {
+            const cx<R> f = run_factor<R, HasQ, HasP, HasS>(hp, k, q, sm);
+            a[k] = cmul(a[k], f);
+            if constexpr (NS == 2) b[k] = cmul(b[k], f);
+        }
+    }
+}
+template <class R, int NR, int NS>
+__device__ __forceinline__ void run_finish_any(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR],
+                                               cx<R> (&b)[NR]) {
+    switch (flags & (kRunHasQ | kRunHasP | kRunHasS)) {
+    case kRunHasS: run_finish_t<R, NR, NS, false, false, true>(hp, q, sm, a, b); break;
+    case kRunHasP: run_finish_t<R, NR, NS, false, true, false>(hp, q, sm, a, b); break;
+    case kRunHasP | kRunHasS: run_finish_t<R, NR, NS, false, true, true>(hp, q, sm, a, b); break;
+    case kRunHasQ: run_finish_t<R, NR, NS, true, false, false>(hp, q, sm, a, b); break;
+    case kRunHasQ | kRunHasS: run_finish_t<R, NR, NS, true, false, true>(hp, q, sm, a, b); break;
+    case kRunHasQ | kRunHasP: run_finish_t<R, NR, NS, true, true, false>(hp, q, sm, a, b); break;
+    case kRunHasQ | kRunHasP | kRunHasS: run_finish_t<R, NR, NS, true, true, true>(hp, q, sm, a, b); break;
+    default: break; // no factor
     }
 }
 template <class R, int NR>
 __device__ __forceinline__ void run_finish(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR]) {
-    if (flags & kRunHasQ)
-        run_finish_q<R, NR, true>(hp, flags, q, sm, a);
-    else
-        run_finish_q<R, NR, false>(hp, flags, q, sm, a);
-}
-template <class R, int NR, bool HasQ, int K = 0>
-__device__ __forceinline__ void run_finish2_q(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR],
-                                              cx<R> (&b)[NR]) {
-    if constexpr (K < NR) {
-        const cx<R> f = run_factor<R, K, HasQ>(hp, flags, q, sm);
-        a[K] = cmul(a[K], f);
-        b[K] = cmul(b[K], f);
-        run_finish2_q<R, NR, HasQ, K + 1>(hp, flags, q, sm, a, b);
-    }
+    run_finish_any<R, NR, 1>(hp, flags, q, sm, a, a);
 }
 template <class R, int NR>
 __device__ __forceinline__ void run_finish2(const CFOp *hp, uint32_t flags, cx<R> q, uint32_t sm, cx<R> (&a)[NR],
                                             cx<R> (&b)[NR]) {
-    if (flags & kRunHasQ)
-        run_finish2_q<R, NR, true>(hp, flags, q, sm, a, b);
-    else
-        run_finish2_q<R, NR, false>(hp, flags, q, sm, a, b);
+    run_finish_any<R, NR, 2>(hp, flags, q, sm, a, b);
 }
 
 // Forward: the run whose header hdr is at hp (Lean: no general members).
@@ -1117,20 +1131,22 @@ __device__ __forceinline__ void diag_run(const CFOp *hp, const FOp &hdr, uint64_
 }
 
 // Overlap sums of the single-state adjoint kernel: acc[slot][column], one
-// column per lane pair, and a sink row that takes the odd lanes' adds (never
-// read), so that adding stays free of divergent branches.
+// column per quad of lanes, and a sink row (one entry per thread, never read)
+// that takes the other three lanes' adds, so that adding stays free of
+// divergent branches.
 struct AccCols {
     double (*acc)[kAccCols];
     double *sink;
 };
-// Adds a lane's overlap to its slot: the lane pair's values are summed first
-// and the even lane adds the sum to the pair's column (its own column: no
+// Adds a lane's overlap to its slot: the quad's values are summed first and
+// its first lane adds the sum to the quad's column (its own column: no
 // contention, program order).
 __device__ __forceinline__ void acc_add(const AccCols &c, int slot, double v) {
-    static_assert(kAccLanes == 2, "one exchange sums a lane pair");
-    const int lane = threadIdx.x & 63, col = threadIdx.x >> 1;
+    static_assert(kAccLanes == 4, "two exchanges sum a lane quad");
+    const int lane = threadIdx.x & 63, col = threadIdx.x >> 2;
     v += xor_lane<0>(v, lane);
-    atomicAdd((lane & 1) ? &c.sink[col] : &c.acc[slot][col], v);
+    v += xor_lane<1>(v, lane);
+    atomicAdd((lane & 3) ? &c.sink[threadIdx.x] : &c.acc[slot][col], v);
 }
 
 // Adjoint: the run applied to lambda (a) and the observable state (b); the
