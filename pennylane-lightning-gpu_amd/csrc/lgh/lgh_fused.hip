@@ -566,11 +566,12 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
 // Three register bits fit three wavefronts per SIMD in the general kernel
 // (at most 168 VGPRs) and four in the lean ones (103-117 VGPRs; 26 KiB of LDS
 // per workgroup); four register bits need all registers of one.  The complex128
-// kernel of lean level 1 is held to 96 VGPRs for five wavefronts per SIMD (a
-// few spills in two rotation branches; measured 1.3 % faster per pass).
+// lean kernels are held to 96 VGPRs for five wavefronts per SIMD (a few spills,
+// mostly outside the gate loop; measured 1.3 % faster per pass at level 1 and
+// 2.2 % at level 2).
 // Lv: lean level of the pass (0: any fusable gate).
 template <class R, int Lv = 0>
-__global__ __launch_bounds__(kBlock, (Lv == 1 && sizeof(R) == 8) ? 5 : 3) void k_fused_adj3(
+__global__ __launch_bounds__(kBlock, (Lv >= 1 && sizeof(R) == 8) ? 5 : 3) void k_fused_adj3(
     const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra, uint64_t ntiles, uint64_t chunk,
     const PassDesc *__restrict__ pd, const FOp *__restrict__ ops, const FGen *__restrict__ gens,
     double *__restrict__ partials, uint64_t bra_zmask, int bra_from_lam) {
