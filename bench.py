@@ -63,7 +63,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=1337)
     ap.add_argument("--tile-bits", type=int, nargs=2, default=None, metavar=("FWD", "ADJ"),
-                    help="register bits of the fused forward / adjoint passes (library default 4 3)")
+                    help="register bits of the fused forward / adjoint passes (library default: 4, and 4 "
+                         "for the adjoint from 24 qubits, else 3; adjoint 0 keeps that default)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default tile heights and grids instead of autotuning them")
     ap.add_argument("--no-batch", action="store_true")
@@ -707,6 +708,15 @@ def main():
         log(f"[bench] autotuning the fused passes on the {n}-qubit headline circuit")
         tune = N.autotune_fused(n, ops)
         log(f"[bench] tuned: forward {tune['forward']}, adjoint {tune['adjoint']}")
+    # register bits of the adjoint passes the timed steps use (the library's
+    # default: 4 for complex128 states of 24 qubits and more, else 3)
+    if tune is not None:
+        adj_rb = int(tune["adjoint"]["tile_bits"])
+    elif args.tile_bits and args.tile_bits[1]:
+        adj_rb = int(args.tile_bits[1])
+    else:
+        adj_rb = 4 if n >= 24 else 3
+    adj_kernel = f"k_fused_adj{adj_rb}<double"  # the lean level follows from the passes
 
     # ---------------------------------------------------------------- headline: adjoint Jacobian
     sv = ops_mod.LightningGPU_C128(n)
@@ -760,7 +770,9 @@ def main():
     if os.path.exists(tfile) and n == 28:
         with open(tfile) as f:
             entry = json.load(f).get(f"k_fused_adj<double>@{n}q", {})
-        if entry.get("kernel_sources_sha16") == kernel_sources_sha16():
+        if entry.get("kernel_sources_sha16") == kernel_sources_sha16() and entry.get("kernels") and \
+                all(k.startswith(adj_kernel) for k in entry["kernels"]):
+            adj_kernel = " + ".join(entry["kernels"])
             traffic = entry.get("traffic_bytes")
             if entry.get("valu_insts"):
                 # VALU issue: a wave64 instruction occupies a SIMD for 4 cycles; 1024 SIMDs
@@ -770,7 +782,8 @@ def main():
                         "issue_ms_at_peak_clock": round(issue_s * 1e3, 4),
                         "issue_frac": round(issue_s / (adj_avg * 1e-3), 3)}
         else:
-            traffic_note = "PMC summary predates the current kernel sources: not used"
+            traffic_note = (f"PMC summary is of other kernels or predates the current kernel sources "
+                            f"({entry.get('kernels')} against {adj_kernel}): not used")
     kernel_stats = {
         "k_fused_adj": {"launches_per_step": adj_n / args.steps, "avg_ms": round(adj_avg, 4),
                         "ms_per_step": round(adj_ms / args.steps, 2), "bytes_per_launch": int(fused_bytes),
@@ -847,8 +860,8 @@ def main():
         "config": {"workload": f"adjoint Jacobian, {n} qubits c128, {layers} layers RY/RZ + CZ brick, "
                                f"{len(ops)} ops, {n_par} params, <Z0 Z{n - 1}>", "seed": args.seed,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_fused_adj3<double, 1> (fused adjoint pass, lean level 1: "
-                                               "lambda + 1 observable state)",
+        "roofline": {"bound": "hbm", "kernel": f"{adj_kernel if adj_kernel.endswith('>') else adj_kernel + ', *>'} "
+                                               "(fused adjoint pass: lambda + 1 observable state)",
                      "achieved": round(fused_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(fused_gbs / HBM_PEAK_GBS, 3), "traffic": traffic, "traffic_note": traffic_note,
                      "bytes_per_launch": int(fused_bytes), "ms_per_launch": round(adj_avg, 4),

@@ -129,7 +129,9 @@ lgh_status lgh_pool_trim(int device);
 lgh_status lgh_set_fusion(int enabled);
 /* Tile height of the fused passes: register bits (3 or 4) of the forward and
  * of the adjoint passes; a wavefront holds 2^(6 + bits) amplitudes per state.
- * Defaults 4 and 3.  No counterpart in the reference; a tuning knob. */
+ * Defaults: forward 4; adjoint 4 for complex128 states of 24 qubits and more,
+ * else 3 (adjoint 0 restores that default).  No counterpart in the
+ * reference; a tuning knob. */
 lgh_status lgh_set_tile_bits(int forward, int adjoint);
 /* Caps on the wavefronts of the fused passes' grids, forward and adjoint
  * (0: the defaults, up to 4096 and 256 wavefronts per CU).  Each wavefront

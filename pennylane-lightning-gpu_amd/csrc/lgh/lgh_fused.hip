@@ -48,7 +48,9 @@ std::atomic<int> g_fusion{1};
 // covers runs of 128 consecutive bytes (8 complex doubles or 16 complex
 // floats), whole cache lines; the other lane positions are chosen per pass.
 int fixed_lanes(lgh_dtype dt) { return dt == LGH_C128 ? 3 : 4; }
-std::atomic<int> g_rb_fwd{kRBForward}, g_rb_adj{kRBAdjoint};
+// adjoint 0: by state size and dtype (adjoint_rb_default)
+std::atomic<int> g_rb_fwd{kRBForward}, g_rb_adj{0};
+int adjoint_rb_default(int n, lgh_dtype dt) { return dt == LGH_C128 && n >= kAdjointWideQubits ? 4 : kRBAdjoint; }
 // passes with only lean gate forms run in the lean kernels (lgh_set_lean_passes)
 std::atomic<bool> g_lean{true};
 // caps on the wavefronts of a pass's grid (0: the per-CU defaults below)
@@ -92,8 +94,9 @@ FusedCfg fused_cfg(int device, int n, lgh_dtype dt, int dir) {
         auto it = g_tuned.find(std::make_tuple(device, n, int(dt), dir));
         if (it != g_tuned.end()) return it->second;
     }
+    const int rb_adj = g_rb_adj.load();
     return dir == 0 ? FusedCfg{g_rb_fwd.load(), kWavesPerCUForward, kMaxSlots}
-                    : FusedCfg{g_rb_adj.load(), kWavesPerCUAdjoint, kMaxSlotsSv};
+                    : FusedCfg{rb_adj ? rb_adj : adjoint_rb_default(n, dt), kWavesPerCUAdjoint, kMaxSlotsSv};
 }
 
 struct Use {
@@ -116,7 +119,8 @@ Use uses_of(const KOp &k) {
 
 void set_fusion(int enabled) { g_fusion.store(enabled ? 1 : 0); }
 void set_tile_bits(int forward, int adjoint) {
-    LGH_REQUIRE((forward == 3 || forward == 4) && (adjoint == 3 || adjoint == 4), "tile bits are 3 or 4");
+    LGH_REQUIRE((forward == 3 || forward == 4) && (adjoint == 0 || adjoint == 3 || adjoint == 4),
+                "tile bits are 3 or 4 (adjoint 0: the default by state size)");
     g_rb_fwd.store(forward);
     g_rb_adj.store(adjoint);
     clear_tuned();
@@ -606,7 +610,7 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
 
 // Three register bits fit three wavefronts per SIMD in the general kernel
 // (at most 168 VGPRs) and four in the lean ones (103-117 VGPRs; 26 KiB of LDS
-// per workgroup); four register bits need all registers of one.  The complex128
+// per workgroup).  The complex128
 // lean kernels are held to 96 VGPRs for five wavefronts per SIMD (a few spills,
 // mostly outside the gate loop; measured 1.3 % faster per pass at level 1 and
 // 2.2 % at level 2).
@@ -618,8 +622,11 @@ __global__ __launch_bounds__(kBlock, (Lv >= 1 && sizeof(R) == 8) ? 5 : 3) void k
     double *__restrict__ partials, uint64_t bra_zmask, int bra_from_lam) {
     fused_adj_body<R, 3, Lv>(lam_in, lam, bra, ntiles, chunk, pd, ops, gens, partials, bra_zmask, bra_from_lam);
 }
+// Four register bits: the complex128 lean kernels are held to 168 VGPRs for
+// three wavefronts per SIMD (39 dwords spilled to scratch at level 1; 4.50
+// against 4.88 ms per 28-qubit pass at two wavefronts).
 template <class R, int Lv = 0>
-__global__ __launch_bounds__(kBlock) void k_fused_adj4(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
+__global__ __launch_bounds__(kBlock, (Lv >= 1 && sizeof(R) == 8) ? 3 : 1) void k_fused_adj4(const cx<R> *lam_in, cx<R> *lam, cx<R> *__restrict__ bra,
                                                     uint64_t ntiles, uint64_t chunk, const PassDesc *__restrict__ pd,
                                                     const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                     double *__restrict__ partials, uint64_t bra_zmask,

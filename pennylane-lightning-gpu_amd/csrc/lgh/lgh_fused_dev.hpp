@@ -34,6 +34,10 @@ constexpr int kAccCols = 256 / kAccLanes;        // columns per workgroup
 constexpr int kMaxSlotsSv = 2 * kMaxSlots;
 constexpr int kRBForward = 4;  // forward passes (one state)
 constexpr int kRBAdjoint = 3;  // adjoint passes (lambda and one observable state)
+// Complex128 adjoint passes from this many qubits on take four register bits
+// by default: a pass then holds more gates, and the sweep of the 28-qubit
+// brick circuit takes 8 passes of 4.50 ms instead of 10 of 4.14 ms.
+constexpr int kAdjointWideQubits = 24;
 
 // Device description of one fused gate (22 dwords, loaded whole and
 // prefetched one gate ahead).  Controls and parity bits among the register

@@ -490,7 +490,7 @@ def test_fused_diagonal_runs_match_oracle(n, bits):
                                    [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
         close64(got64, ref)
     finally:
-        N.set_tile_bits(4, 3)
+        N.set_tile_bits(4, 0)
 
 
 def controlled_fusable_circuit(n, rng, count):
@@ -538,7 +538,7 @@ def test_fused_reordered_passes_match_oracle(n, bits):
                                    [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
         close64(got64, ref)
     finally:
-        N.set_tile_bits(4, 3)
+        N.set_tile_bits(4, 0)
 
 
 @pytest.mark.parametrize("seed", range(6))
@@ -639,7 +639,7 @@ def test_fused_tile_bits_variants(bits):
                                    [N.State(n, np.complex64).upload(b.astype(np.complex64)) for b in bras], ops, tp)
         close64(got64, ref)
     finally:
-        N.set_tile_bits(4, 3)
+        N.set_tile_bits(4, 0)
 
 
 @pytest.mark.parametrize("chunk", [2, 3, 8])
