@@ -74,7 +74,7 @@ constexpr int kMaxWavesPerCUAdjoint = 512;
 struct FusedCfg {
     int rb;
     int waves_per_cu;
-    int slots; // trainable gates per adjoint pass (<= kMaxSlotsSv); unused forward
+    int slots; // trainable gates per adjoint pass (<= adj_slot_cap(rb)); unused forward
 };
 std::mutex g_tune_mu;
 std::map<std::tuple<int, int, int, int>, FusedCfg> g_tuned;
@@ -94,9 +94,9 @@ FusedCfg fused_cfg(int device, int n, lgh_dtype dt, int dir) {
         auto it = g_tuned.find(std::make_tuple(device, n, int(dt), dir));
         if (it != g_tuned.end()) return it->second;
     }
-    const int rb_adj = g_rb_adj.load();
+    const int rb_adj = g_rb_adj.load() ? g_rb_adj.load() : adjoint_rb_default(n, dt);
     return dir == 0 ? FusedCfg{g_rb_fwd.load(), kWavesPerCUForward, kMaxSlots}
-                    : FusedCfg{rb_adj ? rb_adj : adjoint_rb_default(n, dt), kWavesPerCUAdjoint, kMaxSlotsSv};
+                    : FusedCfg{rb_adj, kWavesPerCUAdjoint, adj_slot_cap(rb_adj)};
 }
 
 struct Use {
@@ -488,7 +488,7 @@ __device__ __forceinline__ void fused_adj_body(const cx<R> *lam_in, cx<R> *lam, 
                                                const FOp *__restrict__ ops, const FGen *__restrict__ gens,
                                                double *__restrict__ partials, uint64_t bra_zmask, int bra_from_lam) {
     static_assert(kAccCols * kAccLanes == kBlock, "one accumulator column per lane quad");
-    __shared__ double acc[kMaxSlotsSv][kAccCols];
+    __shared__ double acc[adj_slot_cap(RB)][kAccCols];
     __shared__ double sink[kBlock];
     const AccCols cols{acc, sink};
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1155,7 +1155,7 @@ FusionPlan plan_passes(const lgh_op *ops, int64_t num_ops, int n, bool adjoint, 
     const FusedCfg cfg = fused_cfg(0, n, dtype, adjoint ? 1 : 0);
     const int rb = cfg.rb;
     const auto passes = schedule_passes_cached(kops, n, adjoint ? slot : std::vector<int>{}, rb, fixed_lanes(dtype),
-                                               adjoint ? cfg.slots : kMaxSlots);
+                                               adjoint ? std::min(cfg.slots, adj_slot_cap(rb)) : kMaxSlots);
     FusionPlan plan{};
     for (const Pass &p : *passes) {
         ++plan.passes;
@@ -1194,7 +1194,9 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
     const int n = lam->n;
     const FusedCfg cfg = fused_cfg(lam->device, n, lam->dtype, 1);
     const int rb = cfg.rb;
-    const auto passes = schedule_passes_cached(udag, n, slot_col, rb, fixed_lanes(lam->dtype), cfg.slots);
+    // the kernel's LDS holds adj_slot_cap(rb) slots
+    const auto passes = schedule_passes_cached(udag, n, slot_col, rb, fixed_lanes(lam->dtype),
+                                               std::min(cfg.slots, adj_slot_cap(rb)));
     const Geometry geo = geometry(n, lam->device, rb, cfg.waves_per_cu, g_waves_adj.load());
     // lambda still to be read from lam_src (until the first pass has run)
     const void *src = lam_src;
@@ -1217,6 +1219,7 @@ static void adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std
             continue;
         }
         Built b = build_pass(p, udag, n, true, &gens, &slot_col, rb, true, true, lam->dtype == LGH_C64);
+        LGH_REQUIRE(b.pd.nslots <= adj_slot_cap(rb), "fused adjoint pass: more trainable gates than LDS slots");
         std::vector<int64_t> cols;
         std::vector<double> facs;
         for (size_t s = 0; s < b.slot_ops.size(); ++s) {
@@ -1274,7 +1277,8 @@ void fused_adjoint_one(lgh_sv *lam, const void *lam_src, lgh_sv *bra, const std:
 size_t fused_partials_bytes(int n, int device) {
     // sized for the smaller tile, which has more workgroups
     if (n < kLaneBits + 3) return 0;
-    return sizeof(double) * size_t(kMaxSlotsSv) * geometry(n, device, 3, kMaxWavesPerCUAdjoint).blocks;
+    // the most slots of any tile height times the most blocks (three register bits)
+    return sizeof(double) * size_t(kMaxSlotsSv4) * geometry(n, device, 3, kMaxWavesPerCUAdjoint).blocks;
 }
 
 } // namespace lgh
@@ -1445,8 +1449,8 @@ extern "C" lgh_status lgh_autotune_fused(int device, int32_t num_qubits, lgh_dty
         // more work per pass)
         const FusedCfg fwd_c[] = {{3, 512, kMaxSlots}, {3, 4096, kMaxSlots}, {4, 512, kMaxSlots}, {4, 4096, kMaxSlots}};
         const FusedCfg adj_c[] = {{3, 128, kMaxSlotsSv}, {3, 256, kMaxSlotsSv}, {3, 512, kMaxSlotsSv},
-                                  {3, 256, kMaxSlotsSv / 2}, {3, 512, kMaxSlotsSv / 2}, {4, 128, kMaxSlotsSv},
-                                  {4, 256, kMaxSlotsSv}, {4, 512, kMaxSlotsSv}};
+                                  {3, 256, kMaxSlotsSv / 2}, {4, 256, kMaxSlotsSv}, {4, 128, kMaxSlotsSv4},
+                                  {4, 256, kMaxSlotsSv4}, {4, 512, kMaxSlotsSv4}};
         const FusedCfg def_f = fused_cfg(device, n, dtype, 0), def_a = fused_cfg(device, n, dtype, 1);
         FusedCfg best_f = def_f, best_a = def_a;
         double tbest_f = 1e300, tbest_a = 1e300;

@@ -31,7 +31,11 @@ constexpr int kLaneBits = 6;
 constexpr int kMaxSlots = 24;
 constexpr int kAccLanes = 4;                     // lanes per accumulator column (single-state kernel)
 constexpr int kAccCols = 256 / kAccLanes;        // columns per workgroup
-constexpr int kMaxSlotsSv = 2 * kMaxSlots;
+constexpr int kMaxSlotsSv = 2 * kMaxSlots;  // three register bits: 26 KiB per workgroup
+// Four register bits run at most three workgroups per CU (register-bound), so
+// their passes can hold twice the trainable gates in 50 KiB of LDS.
+constexpr int kMaxSlotsSv4 = 4 * kMaxSlots;
+__host__ __device__ constexpr int adj_slot_cap(int rb) { return rb >= 4 ? kMaxSlotsSv4 : kMaxSlotsSv; }
 constexpr int kRBForward = 4;  // forward passes (one state)
 constexpr int kRBAdjoint = 3;  // adjoint passes (lambda and one observable state)
 // Complex128 adjoint passes from this many qubits on take four register bits
